@@ -1,0 +1,368 @@
+"""TEST INFRASTRUCTURE — the oracle.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
+leg may import this package, and only as the checker. The product path
+(``rexlib_amd``) never imports it.
+
+Contents:
+
+* ``insert_f64``: the double-precision brute-force restatement of the
+  insertion contract (``oracle/insertion_oracle.c``; reference
+  ``include/rexlib/em/ops/reconstruction/insert_slices_operation.hpp:31-112``
+  and the fixture oracle
+  ``tests/unitary/src/backends/cpu/builders/em/reconstruction/fixtures/slice_insertion_cases.hpp:396-563``).
+* ``ref_insert``: the reference's own CPU row kernel, compiled from its sources
+  into ``oracle/_ref/librexlib_ref.so`` (float32 or float64, tiled and threaded
+  like ``slice_insertion_functor.inl:105-156``).
+* ``expand_batch``: the batch semantics of the insertion layout plan
+  (``src/backends/cpu/plans/slice_insertion_layout_plan.cpp:81-185``):
+  numpy broadcasting of the batch prefixes, the axes the accumulator lacks (or
+  holds at extent 1) summed.
+* FFTs through ``numpy.fft`` (pocketfft, the reference's FFT library; scale
+  conventions of ``src/ops/fourier/fourier_normalization.cpp:56-71``).
+* Closed forms for the steps the reference has no function for: the weight
+  quotient (``tests/integration/src/em/functional/test_cpu_reconstruction.cpp:457-464``),
+  gridding correction, per-shell sums and FSC.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+PROFILES = {"box": 0, "triangle": 1, "kaiser_bessel": 2}
+VARIANTS = {"plain": 0, "weighted": 1, "ctf": 2}
+
+
+@dataclass(frozen=True)
+class Window:
+    """A gridding window (include/rexlib/em/ops/parameters/gridding_window.hpp:23-90)."""
+
+    profile: str
+    half_width: float
+    shape: float = 0.0
+
+    @staticmethod
+    def nearest() -> "Window":
+        return Window("box", 0.5)
+
+    @staticmethod
+    def linear() -> "Window":
+        return Window("triangle", 1.0)
+
+    @staticmethod
+    def kaiser_bessel(half_width: float, shape: float) -> "Window":
+        return Window("kaiser_bessel", float(half_width), float(shape))
+
+
+@dataclass(frozen=True)
+class Optics:
+    """CTF optics (include/rexlib/em/ops/parameters/ctf_optics.hpp)."""
+
+    voltage: float = 300.0
+    spherical_aberration: float = 2.7
+    amplitude_contrast: float = 0.1
+
+
+def _load(path):
+    if not os.path.exists(path):
+        return None
+    return ctypes.CDLL(path)
+
+
+_oracle = None
+_ref = None
+
+
+def _lib():
+    global _oracle
+    if _oracle is None:
+        _oracle = _load(os.path.join(_HERE, "liboracle.so"))
+        if _oracle is None:
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        d = ctypes.c_double
+        i64 = ctypes.c_int64
+        p = ctypes.c_void_p
+        _oracle.oracle_insert_f64.argtypes = [ctypes.c_int, ctypes.c_int, d, d, i64, i64, i64, i64,
+                                              p, p, p, p, p, p, d, d, d, d]
+        _oracle.oracle_insert_f64.restype = None
+        _oracle.oracle_bessel_i0.argtypes = [d]
+        _oracle.oracle_bessel_i0.restype = d
+        _oracle.oracle_wavelength.argtypes = [d]
+        _oracle.oracle_wavelength.restype = d
+        _oracle.oracle_window_weight.argtypes = [ctypes.c_int, d, d, d]
+        _oracle.oracle_window_weight.restype = d
+        _oracle.oracle_ctf.argtypes = [p, i64, d, d, d, d, d, d]
+        _oracle.oracle_ctf.restype = d
+    return _oracle
+
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", "librexlib_ref.so"))
+
+
+def _reflib():
+    global _ref
+    if _ref is None:
+        _ref = _load(os.path.join(_HERE, "_ref", "librexlib_ref.so"))
+        if _ref is None:
+            raise RuntimeError("oracle/_ref/librexlib_ref.so is not built")
+        d = ctypes.c_double
+        i64 = ctypes.c_int64
+        p = ctypes.c_void_p
+        f = ctypes.c_float
+        _ref.ref_insert.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, d, d, i64, i64, i64, i64,
+                                    p, p, p, p, p, p, d, d, d, d, ctypes.c_int, i64]
+        _ref.ref_insert.restype = ctypes.c_int
+        _ref.ref_bessel_i0.argtypes = [d]
+        _ref.ref_bessel_i0.restype = d
+        _ref.ref_wavelength.argtypes = [d]
+        _ref.ref_wavelength.restype = d
+        _ref.ref_ctf_f32.argtypes = [d, d, d, d, i64, d, d, d, d, f, f]
+        _ref.ref_ctf_f32.restype = f
+        _ref.ref_kaiser_bessel_f32.argtypes = [d, d, f]
+        _ref.ref_kaiser_bessel_f32.restype = f
+        _ref.ref_axis_phase_f32.argtypes = [p, i64, i64, d]
+        _ref.ref_axis_phase_f32.restype = None
+        _ref.ref_choose_tile_extent.argtypes = [i64, i64, i64, i64, d]
+        _ref.ref_choose_tile_extent.restype = i64
+    return _ref
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------------------
+# Scalar closed forms
+# ---------------------------------------------------------------------------
+
+def bessel_i0(x: float) -> float:
+    return _lib().oracle_bessel_i0(float(x))
+
+
+def wavelength(voltage_kv: float) -> float:
+    return _lib().oracle_wavelength(float(voltage_kv))
+
+
+def window_weight(window: Window, distance: float) -> float:
+    return _lib().oracle_window_weight(PROFILES[window.profile], window.half_width, window.shape,
+                                       float(distance))
+
+
+def ctf(params, extent, pixel_size, optics: Optics, k_x, k_y) -> float:
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    return _lib().oracle_ctf(_ptr(p), int(extent), float(pixel_size), optics.voltage,
+                             optics.spherical_aberration, optics.amplitude_contrast,
+                             float(k_x), float(k_y))
+
+
+# ---------------------------------------------------------------------------
+# Batch semantics of the insertion (slice_insertion_layout_plan.cpp:81-185)
+# ---------------------------------------------------------------------------
+
+CORE_RANKS = {
+    "plain": (3, 3, 2, 2, 1),        # num, den, slices, transforms, radial
+    "weighted": (3, 3, 2, 2, 2, 1),  # num, den, slices, weights, transforms, radial
+    "ctf": (3, 3, 2, 2, 1, 1),       # num, den, slices, transforms, ctf, radial
+}
+
+
+def expand_batch(variant: str, inputs):
+    """Broadcast the batch prefixes of the inputs and split kept from reduced.
+
+    Returns (kept_shape, reduced_count, stacks) where stacks[i] is input i with
+    its batch flattened to (kept, reduced, *core) for i >= 2 and
+    (kept, *core) for the accumulators.
+    """
+    ranks = CORE_RANKS[variant]
+    prefixes = [a.shape[: a.ndim - r] for a, r in zip(inputs, ranks)]
+    frame = np.broadcast_shapes(*prefixes) if prefixes else ()
+    acc_prefix = prefixes[0]
+    padded = (1,) * (len(frame) - len(acc_prefix)) + tuple(acc_prefix)
+    kept_axes = [ax for ax in range(len(frame)) if not (padded[ax] == 1 and frame[ax] > 1)]
+    reduced_axes = [ax for ax in range(len(frame)) if padded[ax] == 1 and frame[ax] > 1]
+    kept_shape = tuple(frame[ax] for ax in kept_axes)
+    reduced_count = int(np.prod([frame[ax] for ax in reduced_axes], dtype=np.int64))
+    order = kept_axes + reduced_axes
+    stacks = []
+    for i, (a, r) in enumerate(zip(inputs, ranks)):
+        core = a.shape[a.ndim - r:]
+        b = np.broadcast_to(a, tuple(frame) + tuple(core))
+        b = np.transpose(b, order + list(range(len(frame), len(frame) + r)))
+        kept_count = int(np.prod(kept_shape, dtype=np.int64))
+        if i < 2:
+            # Every reduced index of an accumulator is the same element.
+            b = b.reshape((kept_count, reduced_count) + tuple(core))[:, 0]
+        else:
+            b = b.reshape((kept_count, reduced_count) + tuple(core))
+        stacks.append(np.ascontiguousarray(b))
+    return kept_shape, reduced_count, stacks
+
+
+def _split(variant, inputs):
+    if variant == "plain":
+        num, den, slices, transforms, radial = inputs
+        extra = None
+    elif variant == "weighted":
+        num, den, slices, extra, transforms, radial = inputs
+    else:
+        num, den, slices, transforms, extra, radial = inputs
+    return num, den, slices, transforms, radial, extra
+
+
+def _run_expanded(variant, inputs, fn):
+    """Expand the batch, run fn over contiguous K x B stacks, restore shapes."""
+    kept_shape, reduced, stacks = expand_batch(variant, inputs)
+    num, den, slices, transforms, radial, extra = _split(variant, stacks)
+    K = num.shape[0]
+    out_num, out_den = fn(K, reduced, num, den, slices, transforms, radial, extra)
+    # The kept axes are exactly the accumulator's own batch axes.
+    return out_num.reshape(inputs[0].shape), out_den.reshape(inputs[1].shape)
+
+
+def insert_f64(variant: str, inputs, window: Window, optics: Optics = Optics(), pixel_size: float = 1.0):
+    """Brute-force FP64 insertion; ``inputs`` in the operation's operand order."""
+    lib = _lib()
+
+    def fn(K, B, num, den, slices, transforms, radial, extra):
+        n = slices.shape[-2]
+        big = num.shape[-3]
+        num = np.ascontiguousarray(num, dtype=np.complex128).copy()
+        den = np.ascontiguousarray(den, dtype=np.float64).copy()
+        sl = np.ascontiguousarray(slices, dtype=np.complex128)
+        tr = np.ascontiguousarray(transforms, dtype=np.float64)
+        ra = np.ascontiguousarray(radial, dtype=np.float64)
+        ex = None if extra is None else np.ascontiguousarray(extra, dtype=np.float64)
+        lib.oracle_insert_f64(VARIANTS[variant], PROFILES[window.profile], window.half_width,
+                              window.shape, n, big, K, B, _ptr(num), _ptr(den), _ptr(sl), _ptr(tr),
+                              _ptr(ra), _ptr(ex), optics.voltage, optics.spherical_aberration,
+                              optics.amplitude_contrast, float(pixel_size))
+        return num, den
+
+    return _run_expanded(variant, inputs, fn)
+
+
+def ref_insert(variant: str, inputs, window: Window, optics: Optics = Optics(), pixel_size: float = 1.0,
+               dtype=np.float32, threads: int = 1, tile_budget: int = 1 << 20):
+    """The reference's own row kernel, in float32 or float64."""
+    lib = _reflib()
+    real = np.dtype(dtype)
+    cplx = np.complex64 if real == np.float32 else np.complex128
+
+    def fn(K, B, num, den, slices, transforms, radial, extra):
+        n = slices.shape[-2]
+        big = num.shape[-3]
+        num = np.ascontiguousarray(num, dtype=cplx).copy()
+        den = np.ascontiguousarray(den, dtype=real).copy()
+        sl = np.ascontiguousarray(slices, dtype=cplx)
+        tr = np.ascontiguousarray(transforms, dtype=real)
+        ra = np.ascontiguousarray(radial, dtype=real)
+        ex = None if extra is None else np.ascontiguousarray(extra, dtype=real)
+        rc = lib.ref_insert(0 if real == np.float32 else 1, VARIANTS[variant], PROFILES[window.profile],
+                            window.half_width, window.shape, n, big, K, B, _ptr(num), _ptr(den), _ptr(sl),
+                            _ptr(tr), _ptr(ra), _ptr(ex), optics.voltage, optics.spherical_aberration,
+                            optics.amplitude_contrast, float(pixel_size), int(threads), int(tile_budget))
+        if rc != 0:
+            raise RuntimeError("ref_insert failed")
+        return num, den
+
+    return _run_expanded(variant, inputs, fn)
+
+
+# ---------------------------------------------------------------------------
+# FFTs (pocketfft through numpy), scale conventions of fourier_normalization.cpp:56-71
+# ---------------------------------------------------------------------------
+
+def rfft2(images):
+    """Real 2-D DFT over the last two axes, unscaled (backward norm)."""
+    return np.fft.rfft2(images, axes=(-2, -1))
+
+
+def irfft3(spectrum, extent):
+    """Inverse real 3-D DFT over the last three axes, scaled by 1/N^3."""
+    return np.fft.irfftn(spectrum, s=(extent, extent, extent), axes=(-3, -2, -1))
+
+
+# ---------------------------------------------------------------------------
+# Closed forms downstream of the insertion
+# ---------------------------------------------------------------------------
+
+def normalize(num, den, threshold=1e-3):
+    """num / den where den > threshold, else 0
+    (tests/integration/src/em/functional/test_cpu_reconstruction.cpp:457-464)."""
+    out = np.zeros_like(num)
+    mask = den > threshold
+    out[mask] = num[mask] / den[mask]
+    return out
+
+
+def window_transform(window: Window, extent: int):
+    """The Fourier transform of the 1-D window sampled at the real-space
+    positions x - C of a volume of extent N, normalised to 1 at 0.
+
+    box: sinc(x / N); triangle: sinc^2(x / N); Kaiser-Bessel: the closed-form
+    transform sinh(sqrt(b^2 - (2 pi w u)^2)) / sqrt(...) with u = x / N.
+    """
+    x = np.arange(extent, dtype=np.float64) - extent // 2
+    u = x / extent
+    if window.profile == "box":
+        return np.sinc(u)
+    if window.profile == "triangle":
+        return np.sinc(u) ** 2
+    w, b = window.half_width, window.shape
+    z2 = b * b - (2 * math.pi * w * u) ** 2
+
+    def kb(zz):
+        zz = np.asarray(zz, dtype=np.float64)
+        out = np.empty_like(zz)
+        pos = zz > 1e-12
+        neg = zz < -1e-12
+        mid = ~(pos | neg)
+        r = np.sqrt(np.abs(zz))
+        out[pos] = np.sinh(r[pos]) / r[pos]
+        out[neg] = np.sin(r[neg]) / r[neg]
+        out[mid] = 1.0
+        return out
+
+    return kb(z2) / kb(np.array([b * b]))[0]
+
+
+def gridding_correct(volume, window: Window):
+    """Divide a real-space volume (origin at C) by the window's transform per axis."""
+    extent = volume.shape[-1]
+    c = window_transform(window, extent)
+    return volume / (c[:, None, None] * c[None, :, None] * c[None, None, :])
+
+
+def shell_index(extent):
+    """Shell of every element of a half spectrum (N, N, N/2+1): round-half-up of |g|."""
+    f = np.fft.fftfreq(extent) * extent
+    fx = np.arange(extent // 2 + 1)
+    r = np.sqrt(f[:, None, None] ** 2 + f[None, :, None] ** 2 + fx[None, None, :] ** 2)
+    return np.floor(r + 0.5).astype(np.int64)
+
+
+def fsc(a, b, extent):
+    """Fourier shell correlation of two half spectra, per shell, in double."""
+    shells = shell_index(extent).ravel()
+    a = a.reshape(-1).astype(np.complex128)
+    b = b.reshape(-1).astype(np.complex128)
+    count = extent // 2 + 1
+    keep = shells < count
+    s = shells[keep]
+    cross = np.bincount(s, weights=(a[keep] * np.conj(b[keep])).real, minlength=count)
+    aa = np.bincount(s, weights=np.abs(a[keep]) ** 2, minlength=count)
+    bb = np.bincount(s, weights=np.abs(b[keep]) ** 2, minlength=count)
+    return cross / np.sqrt(np.maximum(aa * bb, 1e-300))
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.complex128).ravel()
+    b = np.asarray(b, dtype=np.complex128).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
