@@ -1,0 +1,242 @@
+/*
+ * rexlib_hip.h — C ABI of the MI355X (gfx950) backend for rexlib's Fourier
+ * slice insertion path.
+ *
+ * Every entry point takes plain device pointers, extents and strides, and a
+ * HIP stream (NULL: the null stream). Work is enqueued on the stream and the
+ * call returns without waiting, like a program launched by a command queue of
+ * the reference's CUDA plugin (plugins/cuda/src/hardware/cuda_command_queue.cpp:102-145).
+ *
+ * Errors follow the reference's conventions, as status codes:
+ *   REXLIB_HIP_INVALID_ARGUMENT  std::invalid_argument (bad operands or parameters,
+ *                                e.g. src/em/ops/reconstruction/insert_ctf_slices_operation.cpp:17-27)
+ *   REXLIB_HIP_UNSUPPORTED       unsupported_operation_error (no program for the
+ *                                operands, src/core/dispatch/program_service.cpp:84-89)
+ *   REXLIB_HIP_RUNTIME_ERROR     a HIP runtime failure (REXLIB_CUDA_CHECK family,
+ *                                docs/agents/cuda-conventions.md:26-28)
+ * rexlib_hip_last_error() returns the message of the calling thread's last error.
+ */
+#ifndef REXLIB_HIP_H
+#define REXLIB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REXLIB_HIP_MAX_RANK 8
+
+typedef enum rexlib_hip_status {
+	REXLIB_HIP_OK = 0,
+	REXLIB_HIP_INVALID_ARGUMENT = 1,
+	REXLIB_HIP_UNSUPPORTED = 2,
+	REXLIB_HIP_RUNTIME_ERROR = 3
+} rexlib_hip_status;
+
+/* Element types (include/rexlib/core/numerical/numerical_type.hpp:18-46, plus
+ * bfloat16, which the reference lacks and the staging forms below use). */
+typedef enum rexlib_hip_dtype {
+	REXLIB_HIP_FLOAT32 = 0,
+	REXLIB_HIP_FLOAT64 = 1,
+	REXLIB_HIP_COMPLEX64 = 2,
+	REXLIB_HIP_COMPLEX128 = 3,
+	REXLIB_HIP_FLOAT16 = 4,
+	REXLIB_HIP_BFLOAT16 = 5
+} rexlib_hip_dtype;
+
+/* A strided view of device memory: the address of element (0, ..., 0),
+ * extents, and strides in elements (0 broadcasts an axis), as rexlib's
+ * strided_layout (include/rexlib/core/layout/strided_layout.hpp:48-205)
+ * with its offset already applied to `data`. */
+typedef struct rexlib_hip_array {
+	void *data;
+	int32_t dtype;
+	int32_t rank;
+	int64_t extents[REXLIB_HIP_MAX_RANK];
+	int64_t strides[REXLIB_HIP_MAX_RANK];
+} rexlib_hip_array;
+
+/* gridding_window (include/rexlib/em/ops/parameters/gridding_window.hpp:23-90,
+ * gridding_profile.hpp:14-44). */
+typedef enum rexlib_hip_profile {
+	REXLIB_HIP_BOX = 0,
+	REXLIB_HIP_TRIANGLE = 1,
+	REXLIB_HIP_KAISER_BESSEL = 2
+} rexlib_hip_profile;
+
+typedef struct rexlib_hip_window {
+	int32_t profile;
+	double half_width; /* 0.5 for box, 1 for triangle */
+	double shape;      /* beta of Kaiser-Bessel, 0 otherwise */
+} rexlib_hip_window;
+
+/* ctf_optics (include/rexlib/em/ops/parameters/ctf_optics.hpp): kV, mm, fraction. */
+typedef struct rexlib_hip_ctf_optics {
+	double voltage;
+	double spherical_aberration;
+	double amplitude_contrast;
+} rexlib_hip_ctf_optics;
+
+/* ------------------------------------------------------------------------- */
+/* Library state                                                             */
+/* ------------------------------------------------------------------------- */
+
+/* The message of the calling thread's last error ("" if none). */
+const char *rexlib_hip_last_error(void);
+
+/* Version of the rexlib contract this backend implements: 0.1.x
+ * (VERSION:1 of the reference). */
+void rexlib_hip_version(int32_t *major, int32_t *minor, int32_t *patch);
+
+/* Number of visible HIP devices; 0 without a GPU, which is not an error
+ * (plugins/cuda/src/hardware/cuda_device_backend.cpp:37-42). */
+int32_t rexlib_hip_device_count(void);
+
+/* ------------------------------------------------------------------------- */
+/* Slice insertion — the hot path                                            */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * Replaces the program behind rexlib::em::insert_slices
+ * (include/rexlib/em/functional/reconstruction.hpp:47-57;
+ *  operation include/rexlib/em/ops/reconstruction/insert_slices_operation.hpp:16-141;
+ *  CPU program src/backends/cpu/builders/em/reconstruction/insert_slices_program_builder.cpp:15-18).
+ *
+ * Operands, in the operation's order: numerator (..., N, N, N/2+1) complex64,
+ * denominator (..., N, N, N/2+1) float32, slices (..., n, n/2+1) complex64,
+ * transforms (..., 3, 4) float32, radial_weights (..., n/2+1) float32. The
+ * outputs numerator_out / denominator_out have the accumulators' shapes; pass
+ * the accumulators themselves to update them in place. Batch axes broadcast;
+ * those the accumulators lack, or hold at extent 1, are summed.
+ */
+int rexlib_hip_insert_slices(
+	const rexlib_hip_array *numerator,
+	const rexlib_hip_array *denominator,
+	const rexlib_hip_array *slices,
+	const rexlib_hip_array *transforms,
+	const rexlib_hip_array *radial_weights,
+	const rexlib_hip_window *window,
+	const rexlib_hip_array *numerator_out,
+	const rexlib_hip_array *denominator_out,
+	void *stream);
+
+/*
+ * Replaces the program behind rexlib::em::insert_weighted_slices
+ * (include/rexlib/em/functional/reconstruction.hpp:88-100;
+ *  operation insert_weighted_slices_operation.hpp:17-78): one more input,
+ * weights (..., n, n/2+1) float32, after the slices.
+ */
+int rexlib_hip_insert_weighted_slices(
+	const rexlib_hip_array *numerator,
+	const rexlib_hip_array *denominator,
+	const rexlib_hip_array *slices,
+	const rexlib_hip_array *weights,
+	const rexlib_hip_array *transforms,
+	const rexlib_hip_array *radial_weights,
+	const rexlib_hip_window *window,
+	const rexlib_hip_array *numerator_out,
+	const rexlib_hip_array *denominator_out,
+	void *stream);
+
+/*
+ * Replaces the program behind rexlib::em::insert_ctf_slices
+ * (include/rexlib/em/functional/reconstruction.hpp:136-149;
+ *  operation insert_ctf_slices_operation.hpp:17-116): one more input,
+ * ctf (..., 4) float32 = [D_u A, D_v A, angle rad, phase rad], after the
+ * transforms; pixel_size must be finite and positive.
+ */
+int rexlib_hip_insert_ctf_slices(
+	const rexlib_hip_array *numerator,
+	const rexlib_hip_array *denominator,
+	const rexlib_hip_array *slices,
+	const rexlib_hip_array *transforms,
+	const rexlib_hip_array *ctf,
+	const rexlib_hip_array *radial_weights,
+	const rexlib_hip_window *window,
+	const rexlib_hip_ctf_optics *optics,
+	double pixel_size,
+	const rexlib_hip_array *numerator_out,
+	const rexlib_hip_array *denominator_out,
+	void *stream);
+
+/*
+ * Fused form of rfft2 followed by insert_ctf_slices for real images
+ * (..., n, n) stored as float32, bfloat16, or float16 (float16 optionally with
+ * a per-image scale (..., ) float32 that multiplies each image). The images are
+ * zero-padded on chip to `padded` (>= n; 0 means n) before the transform, and
+ * the radial weights then have padded/2+1 entries. All arithmetic is float32.
+ * No reference counterpart: it composes rexlib::rfft2
+ * (include/rexlib/functional/fourier.hpp:483) and em::insert_ctf_slices, with
+ * the same result as the two calls on the float32 images.
+ */
+int rexlib_hip_insert_ctf_images(
+	const rexlib_hip_array *numerator,
+	const rexlib_hip_array *denominator,
+	const rexlib_hip_array *images,
+	const rexlib_hip_array *image_scales,
+	int64_t padded,
+	const rexlib_hip_array *transforms,
+	const rexlib_hip_array *ctf,
+	const rexlib_hip_array *radial_weights,
+	const rexlib_hip_window *window,
+	const rexlib_hip_ctf_optics *optics,
+	double pixel_size,
+	void *stream);
+
+/* ------------------------------------------------------------------------- */
+/* Fourier transforms of the path                                            */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * Replaces the rfft program behind rexlib::rfft2
+ * (include/rexlib/functional/fourier.hpp:483; ops::rfft_operation
+ *  include/rexlib/ops/fourier/rfft_operation.hpp:20-72; CPU
+ *  src/backends/cpu/kernels/pocketfft_fourier_transform.cpp:119-140):
+ * images (..., n, n) float32 -> spectra (..., n, n/2+1) complex64, unscaled.
+ */
+int rexlib_hip_rfft2(const rexlib_hip_array *images, const rexlib_hip_array *spectra, void *stream);
+
+/*
+ * Replaces the irfft program behind rexlib::irfft3
+ * (include/rexlib/functional/fourier.hpp:723; ops::irfft_operation
+ *  include/rexlib/ops/fourier/irfft_operation.hpp:22-75; CPU
+ *  pocketfft_fourier_transform.cpp:142-163): spectrum (..., N, N, N/2+1)
+ * complex64 -> volume (..., N, N, N) float32, scaled by 1/N^3. The imaginary
+ * parts of the x = 0 and (even N) x = N/2 planes' self-conjugate elements are
+ * ignored, as a c2r transform does. The input is not modified.
+ */
+int rexlib_hip_irfft3(const rexlib_hip_array *spectrum, const rexlib_hip_array *volume, void *stream);
+
+/* Inverse of rfft2: spectra (..., n, n/2+1) complex64 -> images (..., n, n)
+ * float32, scaled by 1/n^2 (rexlib::irfft2, include/rexlib/functional/fourier.hpp). */
+int rexlib_hip_irfft2(const rexlib_hip_array *spectra, const rexlib_hip_array *images, void *stream);
+
+/* ------------------------------------------------------------------------- */
+/* Reconstruction steps after the insertion (no reference function; the      */
+/* quotient follows tests/integration/src/em/functional/test_cpu_reconstruction.cpp:457-464) */
+/* ------------------------------------------------------------------------- */
+
+/* out = numerator / denominator where denominator > threshold, else 0.
+ * Contiguous (N, N, N/2+1); out may alias numerator. */
+int rexlib_hip_normalize(const rexlib_hip_array *numerator, const rexlib_hip_array *denominator,
+	double threshold, const rexlib_hip_array *out, void *stream);
+
+/* Divide a real volume (N, N, N) with its origin at N/2 by the separable
+ * transform of the window: sinc (box), sinc^2 (triangle), or the Kaiser-Bessel
+ * closed form, each normalised to 1 at the origin. out may alias volume. */
+int rexlib_hip_gridding_correct(const rexlib_hip_array *volume, const rexlib_hip_window *window,
+	const rexlib_hip_array *out, void *stream);
+
+/* Per-shell sums, in float64, of two half spectra a, b (N, N, N/2+1) complex64:
+ * sums has 4 * (N/2+1) doubles = [sum Re(a conj b), sum |a|^2, sum |b|^2, count]
+ * per shell round-half-up(|g|), each element of the stored half counted once.
+ * The caller zeroes `sums` (device memory) before the first call. */
+int rexlib_hip_shell_sums(const rexlib_hip_array *a, const rexlib_hip_array *b, double *sums,
+	void *stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* REXLIB_HIP_H */

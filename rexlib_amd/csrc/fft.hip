@@ -1,0 +1,506 @@
+// Batched FFTs on gfx950: Stockham autosort transforms in LDS.
+//
+// Every kernel stages a group of lines in LDS, runs the radix-4/2/3/5
+// stages there (ping-pong between two LDS buffers, one butterfly per thread
+// per step), and writes the lines back. Real transforms pack two real rows
+// into one complex line and separate the two half spectra afterwards, so a
+// real row costs half a complex transform.
+//
+// Conventions follow pocketfft as rexlib calls it
+// (src/backends/cpu/kernels/pocketfft_fourier_transform.cpp:74-163): the
+// forward transform is unscaled with exp(-2 pi i), the inverse uses
+// exp(+2 pi i) and a scale of 1/n per axis
+// (src/ops/fourier/fourier_normalization.cpp:56-71). A complex-to-real
+// transform ignores the imaginary parts of the self-conjugate elements.
+#include "fft.hpp"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace rexlib_hip
+{
+
+fft_plan make_fft_plan(int64_t n)
+{
+	fft_plan plan{};
+	plan.n = static_cast<int32_t>(n);
+	int64_t rest = n;
+	const int radices[] = {4, 2, 3, 5};
+	// Radix 4 first, then a single 2, then 3 and 5.
+	while (rest % 4 == 0)
+	{
+		plan.radix[plan.stages++] = 4;
+		rest /= 4;
+	}
+	for (const int r : radices)
+	{
+		while (rest % r == 0)
+		{
+			if (plan.stages == max_fft_stages)
+			{
+				fail(REXLIB_HIP_UNSUPPORTED, "fft: length has too many factors");
+			}
+			plan.radix[plan.stages++] = r;
+			rest /= r;
+		}
+	}
+	if (rest != 1)
+	{
+		fail(REXLIB_HIP_UNSUPPORTED,
+			"fft: the gfx950 transforms take lengths whose prime factors are 2, 3, and 5, not " +
+				std::to_string(n));
+	}
+	return plan;
+}
+
+const float2 *twiddles(int64_t n)
+{
+	static std::mutex mutex;
+	static std::map<std::pair<int, int64_t>, float2 *> cache;
+	int device = 0;
+	REXLIB_HIP_CHECK(hipGetDevice(&device));
+	std::lock_guard<std::mutex> lock(mutex);
+	const auto key = std::make_pair(device, n);
+	const auto found = cache.find(key);
+	if (found != cache.end())
+	{
+		return found->second;
+	}
+	std::vector<float2> host(static_cast<std::size_t>(n));
+	for (int64_t m = 0; m < n; ++m)
+	{
+		const double angle = -2.0 * 3.14159265358979323846 * static_cast<double>(m) / static_cast<double>(n);
+		host[m] = make_float2(static_cast<float>(std::cos(angle)), static_cast<float>(std::sin(angle)));
+	}
+	float2 *device_table = nullptr;
+	REXLIB_HIP_CHECK(hipMalloc(&device_table, sizeof(float2) * static_cast<std::size_t>(n)));
+	REXLIB_HIP_CHECK(hipMemcpy(device_table, host.data(), sizeof(float2) * static_cast<std::size_t>(n),
+		hipMemcpyHostToDevice));
+	cache[key] = device_table;
+	return device_table;
+}
+
+namespace
+{
+
+__device__ inline float2 times_i(float2 z, float sign)
+{
+	// (sign * i) * z
+	return make_float2(-sign * z.y, sign * z.x);
+}
+
+// Radix-R DFT in registers with exp(sign 2 pi i q r / R).
+template <int R>
+__device__ inline void dft(float2 *v, float sign);
+
+template <>
+__device__ inline void dft<2>(float2 *v, float)
+{
+	const float2 a = v[0];
+	const float2 b = v[1];
+	v[0] = cadd(a, b);
+	v[1] = csub(a, b);
+}
+
+template <>
+__device__ inline void dft<4>(float2 *v, float sign)
+{
+	const float2 s02 = cadd(v[0], v[2]);
+	const float2 d02 = csub(v[0], v[2]);
+	const float2 s13 = cadd(v[1], v[3]);
+	const float2 d13 = times_i(csub(v[1], v[3]), sign);
+	v[0] = cadd(s02, s13);
+	v[2] = csub(s02, s13);
+	v[1] = cadd(d02, d13);
+	v[3] = csub(d02, d13);
+}
+
+template <>
+__device__ inline void dft<3>(float2 *v, float sign)
+{
+	const float s3 = 0.86602540378443864676f;
+	const float2 t = cadd(v[1], v[2]);
+	const float2 d = times_i(csub(v[1], v[2]), sign * s3);
+	const float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+	v[0] = cadd(v[0], t);
+	v[1] = cadd(m, d);
+	v[2] = csub(m, d);
+}
+
+template <>
+__device__ inline void dft<5>(float2 *v, float sign)
+{
+	const float c1 = 0.30901699437494742410f;  // cos(2 pi / 5)
+	const float c2 = -0.80901699437494742410f; // cos(4 pi / 5)
+	const float s1 = 0.95105651629515357212f;  // sin(2 pi / 5)
+	const float s2 = 0.58778525229247312917f;  // sin(4 pi / 5)
+	const float2 t1 = cadd(v[1], v[4]);
+	const float2 t2 = cadd(v[2], v[3]);
+	const float2 t3 = csub(v[1], v[4]);
+	const float2 t4 = csub(v[2], v[3]);
+	const float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+	const float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+	const float2 b1 = times_i(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), sign);
+	const float2 b2 = times_i(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), sign);
+	v[0] = cadd(v[0], cadd(t1, t2));
+	v[1] = cadd(a1, b1);
+	v[4] = csub(a1, b1);
+	v[2] = cadd(a2, b2);
+	v[3] = csub(a2, b2);
+}
+
+template <int R>
+__device__ inline void stage(const float2 *a, float2 *b, int lines, int n, int line_stride, int ns,
+	const float2 *tw, float sign)
+{
+	const int m = n / R;
+	const int twiddle_step = n / (ns * R);
+	for (int t = threadIdx.x; t < lines * m; t += blockDim.x)
+	{
+		const int line = t / m;
+		const int j = t - line * m;
+		const int k = j % ns;
+		const float2 *src = a + line * line_stride;
+		float2 v[R];
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+		{
+			v[r] = src[j + r * m];
+		}
+#pragma unroll
+		for (int r = 1; r < R; ++r)
+		{
+			float2 w = tw[k * r * twiddle_step];
+			w.y *= -sign; // the table holds exp(-2 pi i m / n)
+			v[r] = cmul(v[r], w);
+		}
+		dft<R>(v, sign);
+		const int d = (j / ns) * ns * R + k;
+		float2 *dst = b + line * line_stride;
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+		{
+			dst[d + r * ns] = v[r];
+		}
+	}
+}
+
+// Transform `lines` lines of LDS buffer a (line stride line_stride); returns
+// the buffer holding the result. sign -1: forward, +1: inverse (unscaled).
+__device__ float2 *block_fft(float2 *a, float2 *b, int lines, int line_stride, const fft_plan &plan,
+	const float2 *tw, float sign)
+{
+	int ns = 1;
+	const int n = plan.n;
+	for (int st = 0; st < plan.stages; ++st)
+	{
+		const int r = plan.radix[st];
+		switch (r)
+		{
+		case 4: stage<4>(a, b, lines, n, line_stride, ns, tw, sign); break;
+		case 2: stage<2>(a, b, lines, n, line_stride, ns, tw, sign); break;
+		case 3: stage<3>(a, b, lines, n, line_stride, ns, tw, sign); break;
+		default: stage<5>(a, b, lines, n, line_stride, ns, tw, sign); break;
+		}
+		__syncthreads();
+		float2 *swap = a;
+		a = b;
+		b = swap;
+		ns *= r;
+	}
+	return a;
+}
+
+template <int KIND>
+__device__ inline float load_real(const void *base, int64_t index);
+
+template <>
+__device__ inline float load_real<real_f32>(const void *base, int64_t index)
+{
+	return static_cast<const float *>(base)[index];
+}
+
+template <>
+__device__ inline float load_real<real_bf16>(const void *base, int64_t index)
+{
+	return __bfloat162float(static_cast<const __hip_bfloat16 *>(base)[index]);
+}
+
+template <>
+__device__ inline float load_real<real_f16>(const void *base, int64_t index)
+{
+	return __half2float(static_cast<const __half *>(base)[index]);
+}
+
+// Rows of real images, two at a time, to half spectra. Images of extent m
+// are zero-padded to the transform length n with their origin m/2 moved to
+// n/2. `rows` counts the padded rows, n per image, in line set `in` of images
+// (middle = image row, outer = image); the output rows follow line set `out`.
+template <int KIND>
+__global__ void rfft_rows_kernel(const void *in, line_set in_rows, int m, const float *scales,
+	int64_t scale_stride, float2 *out, line_set out_rows, fft_plan plan, const float2 *tw, int pairs)
+{
+	extern __shared__ float2 lds[];
+	const int n = plan.n;
+	const int h = n / 2 + 1;
+	const int stride = n + 1;
+	float2 *a = lds;
+	float2 *b = lds + pairs * stride;
+	float2 *twl = lds + 2 * pairs * stride;
+	for (int i = threadIdx.x; i < n; i += blockDim.x)
+	{
+		twl[i] = tw[i];
+	}
+	const int64_t total = out_rows.count; // padded rows of all images
+	const int offset = n / 2 - m / 2;
+	for (int64_t first = blockIdx.x * static_cast<int64_t>(2 * pairs); first < total;
+		first += static_cast<int64_t>(gridDim.x) * 2 * pairs)
+	{
+		for (int t = threadIdx.x; t < pairs * n; t += blockDim.x)
+		{
+			const int p = t / n;
+			const int i = t - p * n;
+			float re = 0.0f;
+			float im = 0.0f;
+			for (int half = 0; half < 2; ++half)
+			{
+				const int64_t row = first + 2 * p + half;
+				float value = 0.0f;
+				if (row < total)
+				{
+					const int64_t image = row / n;
+					const int r = static_cast<int>(row - image * n) - offset;
+					const int c = i - offset;
+					if (r >= 0 && r < m && c >= 0 && c < m)
+					{
+						const int64_t src = line_offset(in_rows, image * m + r) + c * in_rows.s_elem;
+						value = load_real<KIND>(in, src);
+						if (scales != nullptr)
+						{
+							value *= scales[image * scale_stride];
+						}
+					}
+				}
+				if (half == 0)
+				{
+					re = value;
+				}
+				else
+				{
+					im = value;
+				}
+			}
+			a[p * stride + i] = make_float2(re, im);
+		}
+		__syncthreads();
+		const float2 *z = block_fft(a, b, pairs, stride, plan, twl, -1.0f);
+		for (int t = threadIdx.x; t < pairs * h; t += blockDim.x)
+		{
+			const int p = t / h;
+			const int k = t - p * h;
+			const float2 zk = z[p * stride + k];
+			const float2 zm = cconj(z[p * stride + (n - k) % n]);
+			const int64_t row0 = first + 2 * p;
+			if (row0 < total)
+			{
+				const float2 sum = cadd(zk, zm);
+				out[line_offset(out_rows, row0) + k * out_rows.s_elem] = make_float2(0.5f * sum.x, 0.5f * sum.y);
+			}
+			if (row0 + 1 < total)
+			{
+				// (zk - zm) / (2 i)
+				const float2 diff = csub(zk, zm);
+				out[line_offset(out_rows, row0 + 1) + k * out_rows.s_elem] =
+					make_float2(0.5f * diff.y, -0.5f * diff.x);
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// Complex lines from `src` to `dst` (may be the same memory), `group` lines
+// per step; consecutive lines of a group are loaded by consecutive threads.
+__global__ void fft_lines_kernel(const float2 *src, line_set src_lines, float2 *dst, line_set dst_lines,
+	fft_plan plan, const float2 *tw, float sign, float scale, int group)
+{
+	extern __shared__ float2 lds[];
+	const int n = plan.n;
+	const int stride = n + 1;
+	float2 *a = lds;
+	float2 *b = lds + group * stride;
+	float2 *twl = lds + 2 * group * stride;
+	for (int i = threadIdx.x; i < n; i += blockDim.x)
+	{
+		twl[i] = tw[i];
+	}
+	const int64_t total = src_lines.count;
+	for (int64_t first = blockIdx.x * static_cast<int64_t>(group); first < total;
+		first += static_cast<int64_t>(gridDim.x) * group)
+	{
+		const int lines = static_cast<int>(total - first < group ? total - first : group);
+		for (int t = threadIdx.x; t < group * n; t += blockDim.x)
+		{
+			const int e = t / group;
+			const int l = t - e * group;
+			if (l < lines)
+			{
+				a[l * stride + e] = src[line_offset(src_lines, first + l) + e * src_lines.s_elem];
+			}
+		}
+		__syncthreads();
+		const float2 *z = block_fft(a, b, lines, stride, plan, twl, sign);
+		for (int t = threadIdx.x; t < group * n; t += blockDim.x)
+		{
+			const int e = t / group;
+			const int l = t - e * group;
+			if (l < lines)
+			{
+				const float2 v = z[l * stride + e];
+				dst[line_offset(dst_lines, first + l) + e * dst_lines.s_elem] = make_float2(v.x * scale, v.y * scale);
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// Half-spectrum rows, two at a time, to real rows, scaled.
+__global__ void irfft_rows_kernel(const float2 *src, line_set src_rows, float *dst, line_set dst_rows,
+	fft_plan plan, const float2 *tw, float scale, int pairs)
+{
+	extern __shared__ float2 lds[];
+	const int n = plan.n;
+	const int h = n / 2 + 1;
+	const int stride = n + 1;
+	float2 *a = lds;
+	float2 *b = lds + pairs * stride;
+	float2 *twl = lds + 2 * pairs * stride;
+	for (int i = threadIdx.x; i < n; i += blockDim.x)
+	{
+		twl[i] = tw[i];
+	}
+	const int64_t total = src_rows.count;
+	for (int64_t first = blockIdx.x * static_cast<int64_t>(2 * pairs); first < total;
+		first += static_cast<int64_t>(gridDim.x) * 2 * pairs)
+	{
+		for (int t = threadIdx.x; t < pairs * n; t += blockDim.x)
+		{
+			const int p = t / n;
+			const int k = t - p * n;
+			const bool mirrored = k >= h;
+			const int kk = mirrored ? n - k : k;
+			const bool self_conjugate = kk == 0 || (n % 2 == 0 && kk == n / 2);
+			float2 va = make_float2(0.0f, 0.0f);
+			float2 vb = make_float2(0.0f, 0.0f);
+			const int64_t row0 = first + 2 * p;
+			if (row0 < total)
+			{
+				va = src[line_offset(src_rows, row0) + kk * src_rows.s_elem];
+			}
+			if (row0 + 1 < total)
+			{
+				vb = src[line_offset(src_rows, row0 + 1) + kk * src_rows.s_elem];
+			}
+			if (self_conjugate)
+			{
+				va.y = 0.0f;
+				vb.y = 0.0f;
+			}
+			if (mirrored)
+			{
+				va = cconj(va);
+				vb = cconj(vb);
+			}
+			// Z = A + i B
+			a[p * stride + k] = make_float2(va.x - vb.y, va.y + vb.x);
+		}
+		__syncthreads();
+		const float2 *z = block_fft(a, b, pairs, stride, plan, twl, 1.0f);
+		for (int t = threadIdx.x; t < pairs * n; t += blockDim.x)
+		{
+			const int p = t / n;
+			const int i = t - p * n;
+			const float2 v = z[p * stride + i];
+			const int64_t row0 = first + 2 * p;
+			if (row0 < total)
+			{
+				dst[line_offset(dst_rows, row0) + i * dst_rows.s_elem] = v.x * scale;
+			}
+			if (row0 + 1 < total)
+			{
+				dst[line_offset(dst_rows, row0 + 1) + i * dst_rows.s_elem] = v.y * scale;
+			}
+		}
+		__syncthreads();
+	}
+}
+
+unsigned grid_for(int64_t work, int64_t per_block)
+{
+	const int64_t blocks = (work + per_block - 1) / per_block;
+	return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks, 1 << 16)));
+}
+
+int lines_per_group(int n)
+{
+	// About 4096 complex elements (32 KiB) per LDS buffer.
+	return std::max(1, std::min(16, 4096 / std::max(n, 1)));
+}
+
+} // anonymous namespace
+
+void launch_rfft_rows(int kind, const void *in, const line_set &in_rows, int m, const float *scales,
+	int64_t scale_stride, float2 *out, const line_set &out_rows, int64_t n, hipStream_t stream)
+{
+	const fft_plan plan = make_fft_plan(n);
+	const float2 *tw = twiddles(n);
+	const int pairs = std::max(1, lines_per_group(static_cast<int>(n)) / 2);
+	const std::size_t shared = sizeof(float2) * (2 * pairs * (n + 1) + n);
+	const unsigned grid = grid_for(out_rows.count, 2 * pairs);
+	switch (kind)
+	{
+	case real_f32:
+		rfft_rows_kernel<real_f32><<<grid, 256, shared, stream>>>(in, in_rows, m, scales, scale_stride, out,
+			out_rows, plan, tw, pairs);
+		break;
+	case real_bf16:
+		rfft_rows_kernel<real_bf16><<<grid, 256, shared, stream>>>(in, in_rows, m, scales, scale_stride, out,
+			out_rows, plan, tw, pairs);
+		break;
+	default:
+		rfft_rows_kernel<real_f16><<<grid, 256, shared, stream>>>(in, in_rows, m, scales, scale_stride, out,
+			out_rows, plan, tw, pairs);
+		break;
+	}
+	REXLIB_HIP_CHECK(hipGetLastError());
+}
+
+void launch_fft_lines(const float2 *src, const line_set &src_lines, float2 *dst, const line_set &dst_lines,
+	int64_t n, float sign, float scale, hipStream_t stream)
+{
+	const fft_plan plan = make_fft_plan(n);
+	const float2 *tw = twiddles(n);
+	const int group = lines_per_group(static_cast<int>(n));
+	const std::size_t shared = sizeof(float2) * (2 * group * (n + 1) + n);
+	const unsigned grid = grid_for(src_lines.count, group);
+	fft_lines_kernel<<<grid, 256, shared, stream>>>(src, src_lines, dst, dst_lines, plan, tw, sign, scale, group);
+	REXLIB_HIP_CHECK(hipGetLastError());
+}
+
+void launch_irfft_rows(const float2 *src, const line_set &src_rows, float *dst, const line_set &dst_rows,
+	int64_t n, float scale, hipStream_t stream)
+{
+	const fft_plan plan = make_fft_plan(n);
+	const float2 *tw = twiddles(n);
+	const int pairs = std::max(1, lines_per_group(static_cast<int>(n)) / 2);
+	const std::size_t shared = sizeof(float2) * (2 * pairs * (n + 1) + n);
+	const unsigned grid = grid_for(src_rows.count, 2 * pairs);
+	irfft_rows_kernel<<<grid, 256, shared, stream>>>(src, src_rows, dst, dst_rows, plan, tw, scale, pairs);
+	REXLIB_HIP_CHECK(hipGetLastError());
+}
+
+} // namespace rexlib_hip

@@ -1,0 +1,49 @@
+// Host launchers of the kernels, called by the C ABI.
+#pragma once
+
+#include "fft.hpp"
+#include "insertion.hpp"
+
+namespace rexlib_hip
+{
+
+// Workspace areas, one per use, per device and stream.
+enum workspace_area : int
+{
+	area_insertion = 0,
+	area_spectra = 1,
+	area_fft = 2,
+	area_count = 3
+};
+
+void *workspace_area_bytes(int area, std::size_t bytes, hipStream_t stream);
+
+// Insertion schedules.
+enum insertion_schedule : int
+{
+	schedule_auto = 0,
+	schedule_scatter = 1,
+	schedule_brick = 2
+};
+
+void run_insertion(insertion_params p, int schedule, const float2 *num_in, const int64_t *num_in_strides,
+	const float *den_in, const int64_t *den_in_strides, bool in_place, double shape, hipStream_t stream);
+
+void launch_rfft_rows(int kind, const void *in, const line_set &in_rows, int m, const float *scales,
+	int64_t scale_stride, float2 *out, const line_set &out_rows, int64_t n, hipStream_t stream);
+
+void launch_fft_lines(const float2 *src, const line_set &src_lines, float2 *dst, const line_set &dst_lines,
+	int64_t n, float sign, float scale, hipStream_t stream);
+
+void launch_irfft_rows(const float2 *src, const line_set &src_rows, float *dst, const line_set &dst_rows,
+	int64_t n, float scale, hipStream_t stream);
+
+void launch_normalize(const float2 *num, const float *den, float2 *out, int64_t count, double threshold,
+	hipStream_t stream);
+
+void launch_gridding(const float *in, float *out, int big, int64_t volumes, int profile, double half_width,
+	double shape, hipStream_t stream);
+
+void launch_shell_sums(const float2 *a, const float2 *b, double *sums, int big, hipStream_t stream);
+
+} // namespace rexlib_hip

@@ -1,0 +1,141 @@
+"""The C ABI library on the CPU: it loads, exports what include/rexlib_hip.h
+declares, and rejects invalid operands before touching a device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from rexlib_amd import _native as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "rexlib_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(rexlib_hip_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    lib = N.library()
+    declared = declared_functions()
+    assert "rexlib_hip_insert_ctf_slices" in declared
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(N.EXPORTS) == declared
+
+
+def test_version_is_the_reference_contract():
+    major, minor, patch = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    N.library().rexlib_hip_version(ctypes.byref(major), ctypes.byref(minor), ctypes.byref(patch))
+    assert (major.value, minor.value) == (0, 1)
+
+
+def test_no_device_is_not_an_error():
+    assert N.library().rexlib_hip_device_count() >= 0
+
+
+def fake(shape, dtype, address=0x1000):
+    """A descriptor of memory that is never touched: validation fails first."""
+    a = N.Array()
+    a.data = address
+    a.dtype = dtype
+    a.rank = len(shape)
+    stride = 1
+    strides = []
+    for e in reversed(shape):
+        strides.append(stride)
+        stride *= e
+    for i, (e, s) in enumerate(zip(shape, reversed(strides))):
+        a.extents[i] = e
+        a.strides[i] = s
+    return a
+
+
+def call_ctf(num, den, sl, tr, ctf, ra, window=None, optics=None, pixel=1.5):
+    lib = N.library()
+    window = window or N.Window(1, 1.0, 0.0)
+    optics = optics or N.Optics(300.0, 2.7, 0.1)
+    return lib.rexlib_hip_insert_ctf_slices(ctypes.byref(num), ctypes.byref(den), ctypes.byref(sl),
+                                            ctypes.byref(tr), ctypes.byref(ctf), ctypes.byref(ra),
+                                            ctypes.byref(window), ctypes.byref(optics), pixel,
+                                            None, None, None)
+
+
+def good_operands(n=8, big=16, dtype_real=N.FLOAT32, dtype_complex=N.COMPLEX64):
+    h = n // 2 + 1
+    half = big // 2 + 1
+    return [fake((big, big, half), dtype_complex), fake((big, big, half), dtype_real),
+            fake((3, n, h), dtype_complex), fake((3, 3, 4), dtype_real), fake((3, 4), dtype_real),
+            fake((3, h), dtype_real)]
+
+
+def message():
+    return N.library().rexlib_hip_last_error().decode()
+
+
+def test_pixel_size_must_be_positive():
+    # src/em/ops/reconstruction/insert_ctf_slices_operation.cpp:17-27
+    assert call_ctf(*good_operands(), pixel=0.0) == N.INVALID_ARGUMENT
+    assert "pixel size" in message()
+    assert call_ctf(*good_operands(), pixel=float("nan")) == N.INVALID_ARGUMENT
+
+
+def test_optics_are_checked():
+    assert call_ctf(*good_operands(), optics=N.Optics(300.0, 2.7, 1.0)) == N.INVALID_ARGUMENT
+    assert "amplitude contrast" in message()
+    assert call_ctf(*good_operands(), optics=N.Optics(-1.0, 2.7, 0.1)) == N.INVALID_ARGUMENT
+
+
+def test_kaiser_bessel_window_is_checked():
+    assert call_ctf(*good_operands(), window=N.Window(2, -1.0, 2.0)) == N.INVALID_ARGUMENT
+    assert "half-width" in message()
+    assert call_ctf(*good_operands(), window=N.Window(2, 2.0, -1.0)) == N.INVALID_ARGUMENT
+    assert call_ctf(*good_operands(), window=N.Window(2, 5.0, 2.0)) == N.UNSUPPORTED
+
+
+def test_shapes_are_checked():
+    ops = good_operands()
+    ops[2] = fake((3, 8, 4), N.COMPLEX64)  # (n, n//2) instead of (n, n//2+1)
+    assert call_ctf(*ops) == N.INVALID_ARGUMENT
+    assert "slice" in message()
+    ops = good_operands()
+    ops[1] = fake((16, 16, 8), N.FLOAT32)
+    assert call_ctf(*ops) == N.INVALID_ARGUMENT
+    ops = good_operands()
+    ops[4] = fake((3, 5), N.FLOAT32)
+    assert call_ctf(*ops) == N.INVALID_ARGUMENT
+    ops = good_operands()
+    ops[3] = fake((2, 3, 4), N.FLOAT32)  # batch 2 against 3
+    assert call_ctf(*ops) == N.INVALID_ARGUMENT
+    assert "broadcast" in message()
+
+
+def test_types_are_checked_and_float64_is_unsupported():
+    ops = good_operands()
+    ops[2] = fake((3, 8, 5), N.FLOAT32)  # real slices
+    assert call_ctf(*ops) == N.INVALID_ARGUMENT
+    assert call_ctf(*good_operands(dtype_real=N.FLOAT64, dtype_complex=N.COMPLEX128)) == N.UNSUPPORTED
+
+
+def test_fft_lengths_with_other_prime_factors_are_unsupported():
+    lib = N.library()
+    images = fake((2, 7, 7), N.FLOAT32)
+    spectra = fake((2, 7, 4), N.COMPLEX64)
+    assert lib.rexlib_hip_rfft2(ctypes.byref(images), ctypes.byref(spectra), None) == N.UNSUPPORTED
+    assert "prime factors" in message()
+
+
+def test_python_api_rejects_host_tensors():
+    import torch
+
+    from rexlib_amd import em
+    t = torch.zeros(4)
+    with pytest.raises(N.UnsupportedOperationError):
+        em.insert_slices(t, t, t, t, t, em.GriddingWindow.linear())
+    with pytest.raises(ValueError):
+        em.GriddingWindow.kaiser_bessel(0.0, 1.0)
+    with pytest.raises(ValueError):
+        em.CtfOptics(300, 2.7, 1.5)
+    assert em.CtfOptics(300, 2.7, 0.1).wavelength == pytest.approx(0.019687615309, rel=1e-9)

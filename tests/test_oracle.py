@@ -1,0 +1,280 @@
+"""The oracle, pinned before it is trusted (CPU only).
+
+Anchors, each chosen so that a plausible slip (a dropped term, a wrong sign
+or index, a transposed operand) fails one of them:
+  * known answers copied as numbers from the reference's tests;
+  * the reference's own CPU kernels, compiled from its sources into
+    oracle/_ref (when built) and, everywhere, the golden vectors they wrote;
+  * closed forms: the central plane of numpy.fft.rfftn, Hermitian symmetry.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from cases import PIXEL_SIZE, SIZES, identity_transform, make_case, spread_kat_case
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "insertion_reference.npz")
+WINDOWS = {
+    "nearest": oracle.Window.nearest(),
+    "linear": oracle.Window.linear(),
+    "kb1.5_6": oracle.Window.kaiser_bessel(1.5, 6.0),
+    "kb2_8": oracle.Window.kaiser_bessel(2.0, 8.0),
+}
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+
+
+def tolerance_for(window):
+    # slice_insertion_cases.hpp:648-654: 1e-9 for box and triangle, 5e-4 for KB.
+    return 5e-4 if window.profile == "kaiser_bessel" else 1e-9
+
+
+# --- known answers from the reference's tests --------------------------------
+
+def test_wavelength_known_values():
+    # tests/unitary/src/em/ops/parameters/test_ctf_optics.cpp:40-53
+    assert oracle.wavelength(300) == pytest.approx(0.019687615309, rel=1e-9)
+    assert oracle.wavelength(200) == pytest.approx(0.025079493567, rel=1e-9)
+
+
+def test_bessel_i0_known_values():
+    # tests/unitary/src/backends/cpu/kernels/test_bessel_i0.cpp:11-24
+    assert oracle.bessel_i0(0.0) == 1.0
+    for x, v, rel in [(0.5, 1.0634833707413236, 1e-14), (1.0, 1.2660658777520082, 1e-14),
+                      (2.0, 2.279585302336067, 1e-14), (5.0, 27.23987182360445, 1e-14),
+                      (10.0, 2815.716628466255, 1e-13)]:
+        assert oracle.bessel_i0(x) == pytest.approx(v, rel=rel)
+
+
+def test_spread_of_one_sample_over_eight_voxels():
+    # test_row_insertion_kernel.cpp:47-112: a sample at (1.25, 2.5, -0.75), value 2.
+    num, den = oracle.insert_f64("weighted", spread_kat_case(), oracle.Window.linear())
+    assert den[7, 2, 1] == pytest.approx(0.75 * 0.5 * 0.75, abs=1e-12)
+    assert den[0, 3, 2] == pytest.approx(0.25 * 0.5 * 0.25, abs=1e-12)
+    assert num[7, 2, 1].real == pytest.approx(2 * 0.75 * 0.5 * 0.75, abs=1e-12)
+    assert num[0, 3, 2].real == pytest.approx(-2 * 0.25 * 0.5 * 0.25, abs=1e-12)
+    assert num[2, 3, 0].real == pytest.approx(0.0, abs=1e-12)
+    assert den.sum() == pytest.approx(1.0, abs=1e-12)
+    assert np.abs(num.imag).max() < 1e-12
+
+
+def test_ctf_at_zero_frequency_is_the_amplitude_contrast():
+    # tests/python/em/functional/test_reconstruction.py:93-111
+    n = 8
+    slices = np.ones((n, n // 2 + 1), np.complex128)
+    ctf = np.array([15000, 15000, 0, 0], np.float64)
+    acc = np.zeros((n, n, n // 2 + 1), np.complex128)
+    num, den = oracle.insert_f64("ctf", [acc, acc.real.copy(), slices, identity_transform(), ctf,
+                                         np.ones(n // 2 + 1)], oracle.Window.nearest(), pixel_size=1.5)
+    assert den[0, 0, 0] == pytest.approx(0.01)
+    assert num[0, 0, 0].real == pytest.approx(0.1)
+
+
+@pytest.mark.parametrize("n", [7, 8])
+def test_projection_lands_on_the_central_plane(n):
+    # tests/python/em/functional/test_reconstruction.py:12-38 (and the C++
+    # integration test test_cpu_reconstruction.cpp:320-346): nearest window,
+    # identity transform, radial weight 1.
+    volume = np.random.default_rng(1).normal(size=(n, n, n))
+    slices = np.fft.rfft2(volume.sum(axis=0))
+    acc = np.zeros((n, n, n // 2 + 1), np.complex128)
+    num, den = oracle.insert_f64("plain", [acc, acc.real.copy(), slices, identity_transform(),
+                                           np.ones(n // 2 + 1)], oracle.Window.nearest())
+    inserted = den > 0
+    assert np.all(den[inserted] == 1)
+    assert not np.any(inserted[1:])
+    np.testing.assert_allclose(num[inserted], np.fft.rfftn(volume)[inserted], atol=1e-9)
+
+
+def test_projection_along_x_lands_on_the_x_plane():
+    # test_cpu_reconstruction.cpp:348-377: the x-projection under an axis swap.
+    n = 8
+    volume = np.random.default_rng(4).normal(size=(n, n, n))
+    image = volume.sum(axis=2)  # axes (z, y): image x = volume z, image y = volume y
+    slices = np.fft.rfft2(image.T)
+    t = np.array([[0, 0, 1, 0], [0, 1, 0, 0], [1, 0, 0, 0]], np.float64)
+    acc = np.zeros((n, n, n // 2 + 1), np.complex128)
+    num, den = oracle.insert_f64("plain", [acc, acc.real.copy(), slices, t, np.ones(n // 2 + 1)],
+                                 oracle.Window.nearest())
+    full = np.fft.rfftn(volume)
+    inserted = den > 0
+    assert np.all(inserted[..., 1:] == False)  # noqa: E712
+    np.testing.assert_allclose(num[inserted], full[inserted], atol=1e-9)
+
+
+def test_shift_is_undone_by_the_translation():
+    # test_cpu_reconstruction.cpp:379-416: shift an image by (dx, dy) and pass
+    # t = (dx, dy); the result equals the unshifted insertion.
+    n = 8
+    rng = np.random.default_rng(5)
+    image = rng.normal(size=(n, n))
+    shifted = np.roll(image, (1, 2), axis=(0, 1))
+    acc = np.zeros((n, n, n // 2 + 1), np.complex128)
+    t = identity_transform()
+    t2 = t.copy()
+    t2[0, 3] = 2.0
+    t2[1, 3] = 1.0
+    w = oracle.Window.linear()
+    a = oracle.insert_f64("plain", [acc, acc.real.copy(), np.fft.rfft2(image), t, np.ones(n // 2 + 1)], w)
+    b = oracle.insert_f64("plain", [acc, acc.real.copy(), np.fft.rfft2(shifted), t2, np.ones(n // 2 + 1)], w)
+    np.testing.assert_allclose(b[0], a[0], atol=1e-9)
+    np.testing.assert_allclose(b[1], a[1], atol=1e-12)
+
+
+def test_nan_beyond_the_cutoff_is_left_out():
+    # test_slice_insertion_program_builder.cpp:221-246
+    inputs = make_case("plain", 8, 16, 4)
+    inputs[4][:, 3] = 0.0
+    inputs[4][:, 4] = 0.0
+    inputs[2][:, 0, 4] = np.nan + 1j * np.nan
+    num, den = oracle.insert_f64("plain", inputs, oracle.Window.linear())
+    assert not np.isnan(num).any()
+    assert not np.isnan(den).any()
+
+
+def test_batch_axes_the_accumulator_lacks_are_summed():
+    # test_slice_insertion_program_builder.cpp:193-219: accumulators (2, 1),
+    # transforms (2, 3): each accumulator receives three slices.
+    inputs = make_case("plain", 6, 8, 3, accumulator_batch=(2, 1), transform_batch=(2, 3))
+    whole = oracle.insert_f64("plain", inputs, oracle.Window.linear())
+    assert whole[0].shape == (2, 1, 8, 8, 5)
+    for k in range(2):
+        single = [inputs[0][k, 0], inputs[1][k, 0], inputs[2], inputs[3][k], inputs[4]]
+        part = oracle.insert_f64("plain", single, oracle.Window.linear())
+        np.testing.assert_allclose(whole[0][k, 0], part[0], atol=1e-12)
+        np.testing.assert_allclose(whole[1][k, 0], part[1], atol=1e-12)
+
+
+def test_split_batch_equals_whole():
+    # test_cpu_reconstruction.cpp:505-594 and test_reconstruction.py:40-74
+    inputs = make_case("ctf", 8, 16, 7, slices=4, zero_accumulators=True)
+    w = oracle.Window.linear()
+    whole = oracle.insert_f64("ctf", inputs, w, pixel_size=PIXEL_SIZE)
+    first = oracle.insert_f64("ctf", [inputs[0], inputs[1]] + [a[:2] for a in inputs[2:]], w,
+                              pixel_size=PIXEL_SIZE)
+    second = oracle.insert_f64("ctf", [first[0], first[1]] + [a[2:] for a in inputs[2:]], w,
+                               pixel_size=PIXEL_SIZE)
+    np.testing.assert_allclose(second[0], whole[0], atol=1e-12)
+    np.testing.assert_allclose(second[1], whole[1], atol=1e-12)
+
+
+def test_weighted_denominator_equals_the_weight():
+    # tests/python/em/functional/test_reconstruction.py:76-91
+    n = 8
+    slices = np.fft.rfft2(np.random.default_rng(3).normal(size=(n, n)))
+    weights = np.full(slices.shape, 0.25)
+    acc = np.zeros((n, n, n // 2 + 1), np.complex128)
+    _, den = oracle.insert_f64("weighted", [acc, acc.real.copy(), slices, weights, identity_transform(),
+                                            np.ones(n // 2 + 1)], oracle.Window.nearest())
+    inserted = den > 0
+    assert np.all(den[inserted] == 0.25)
+
+
+def test_ctf_formula_matches_the_reference_polynomial_form():
+    # The oracle evaluates the contract's atan2/cos form in double; the
+    # reference evaluates its float32 polynomial (contrast_transfer_function.inl:73-87).
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    lib = oracle._reflib()
+    rng = np.random.default_rng(9)
+    optics = oracle.Optics()
+    for _ in range(200):
+        p = [rng.uniform(8000, 25000), rng.uniform(8000, 25000), rng.uniform(0, math.pi), rng.uniform(0, 0.5)]
+        kx, ky = float(rng.integers(0, 33)), float(rng.integers(-32, 33))
+        a = oracle.ctf(p, 64, 1.5, optics, kx, ky)
+        b = lib.ref_ctf_f32(300.0, 2.7, 0.1, 1.5, 64, *p, kx, ky)
+        assert abs(a - b) < 2e-4
+
+
+def test_kaiser_bessel_matches_the_reference_table():
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    lib = oracle._reflib()
+    w = oracle.Window.kaiser_bessel(2.0, 8.0)
+    for d in np.linspace(-2.5, 2.5, 101):
+        assert abs(oracle.window_weight(w, d) - lib.ref_kaiser_bessel_f32(2.0, 8.0, d)) < 1e-5
+
+
+# --- the reference's kernels --------------------------------------------------
+
+def _golden():
+    return np.load(GOLDEN, allow_pickle=False)
+
+
+def golden_cases():
+    with np.load(GOLDEN, allow_pickle=False) as g:
+        names = sorted({k.split("/")[0] for k in g.files})
+    return names
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_oracle_matches_the_reference_golden_vectors(name):
+    g = _golden()
+    meta = g[f"{name}/meta"]
+    variant = str(g[f"{name}/variant"])
+    window = WINDOWS[str(g[f"{name}/window"])]
+    inputs = [g[f"{name}/in{i}"] for i in range(int(meta[2]))]
+    num, den = oracle.insert_f64(variant, inputs, window, pixel_size=PIXEL_SIZE)
+    tol = tolerance_for(window)
+    np.testing.assert_allclose(num, g[f"{name}/num64"], atol=tol, rtol=0)
+    np.testing.assert_allclose(den, g[f"{name}/den64"], atol=tol, rtol=0)
+    # The reference in float32 stays within the 1e-4 of
+    # test_slice_insertion_program_builder.cpp:190 (KB: 5e-4).
+    tol32 = max(tol, 1e-4)
+    np.testing.assert_allclose(num, g[f"{name}/num32"], atol=tol32, rtol=0)
+    np.testing.assert_allclose(den, g[f"{name}/den32"], atol=tol32, rtol=0)
+
+
+@needs_ref
+@pytest.mark.parametrize("n,big", SIZES)
+@pytest.mark.parametrize("wname", ["nearest", "linear", "kb1.5_6"])
+def test_oracle_matches_the_reference_kernels(n, big, wname):
+    inputs = make_case("plain", n, big, 11)
+    w = WINDOWS[wname]
+    a = oracle.insert_f64("plain", inputs, w)
+    b = oracle.ref_insert("plain", inputs, w, dtype=np.float64, threads=3, tile_budget=1)
+    np.testing.assert_allclose(a[0], b[0], atol=tolerance_for(w), rtol=0)
+    np.testing.assert_allclose(a[1], b[1], atol=tolerance_for(w), rtol=0)
+
+
+@needs_ref
+def test_reference_kernels_agree_across_tilings_and_threads():
+    # The reference's own tile-independence (test_slice_insertion_functor.cpp:87-143).
+    inputs = make_case("ctf", 8, 16, 2)
+    w = oracle.Window.linear()
+    a = oracle.ref_insert("ctf", inputs, w, pixel_size=PIXEL_SIZE, threads=1, tile_budget=1 << 30)
+    b = oracle.ref_insert("ctf", inputs, w, pixel_size=PIXEL_SIZE, threads=4, tile_budget=1)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+# --- closed forms downstream of the insertion ----------------------------------
+
+def test_window_transforms_at_known_points():
+    assert oracle.window_transform(oracle.Window.linear(), 8)[4] == 1.0
+    # triangle: sinc^2(1/4) at x - C = 2 of N = 8
+    assert oracle.window_transform(oracle.Window.linear(), 8)[6] == pytest.approx(
+        (math.sin(math.pi / 4) / (math.pi / 4)) ** 2)
+    kb = oracle.window_transform(oracle.Window.kaiser_bessel(2.0, 8.0), 64)
+    assert kb[32] == pytest.approx(1.0)
+    # The KB transform equals a direct numerical transform of the window.
+    d = np.linspace(-2, 2, 4001)
+    phi = np.array([oracle.window_weight(oracle.Window.kaiser_bessel(2.0, 8.0), x) for x in d])
+    for j in (33, 40, 50):
+        u = (j - 32) / 64
+        direct = np.trapezoid(phi * np.cos(2 * math.pi * u * d), d) / np.trapezoid(phi, d)
+        assert kb[j] == pytest.approx(direct, rel=1e-4, abs=1e-6)
+
+
+def test_fsc_of_a_spectrum_with_itself_is_one():
+    rng = np.random.default_rng(0)
+    a = rng.normal(size=(8, 8, 5)) + 1j * rng.normal(size=(8, 8, 5))
+    assert np.allclose(oracle.fsc(a, a, 8), 1.0)
+    assert np.allclose(oracle.fsc(a, -a, 8), -1.0)
+
+
+def test_normalize_threshold():
+    num = np.array([1 + 1j, 2, 3], np.complex128)
+    den = np.array([2.0, 1e-4, 0.5])
+    np.testing.assert_allclose(oracle.normalize(num, den), [0.5 + 0.5j, 0, 6])

@@ -1,0 +1,110 @@
+"""The whole path on the GPU: rfft2 -> insertion -> normalisation -> irfft3
+-> gridding correction, against the CPU oracle on the same inputs."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import rexlib_amd
+from gpu_helpers import OPTICS, to_device
+from rexlib_amd import em, synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def test_normalize_matches_the_quotient():
+    rng = np.random.default_rng(0)
+    num = (rng.normal(size=(16, 16, 9)) + 1j * rng.normal(size=(16, 16, 9))).astype(np.complex64)
+    den = rng.uniform(-0.01, 0.05, (16, 16, 9)).astype(np.float32)
+    got = em.normalize(to_device(num), to_device(den), 1e-3).cpu().numpy()
+    ref = oracle.normalize(num.astype(np.complex128), den.astype(np.float64), 1e-3)
+    assert np.max(np.abs(got - ref)) <= 1e-6 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("window", [oracle.Window.nearest(), oracle.Window.linear(),
+                                    oracle.Window.kaiser_bessel(2.0, 8.0)])
+@pytest.mark.parametrize("big", [16, 17, 64])
+def test_gridding_correction_matches_the_closed_form(window, big):
+    from gpu_helpers import hip_window
+    x = np.random.default_rng(big).normal(size=(big, big, big)).astype(np.float32)
+    got = em.gridding_correct(to_device(x), hip_window(window)).cpu().numpy()
+    ref = oracle.gridding_correct(x.astype(np.float64), window)
+    assert oracle.rel_l2(got, ref) < 1e-6
+
+
+@pytest.mark.parametrize("big", [16, 33, 64])
+def test_fsc_matches_the_oracle(big):
+    rng = np.random.default_rng(big)
+    shape = (big, big, big // 2 + 1)
+    a = (rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64)
+    b = (a + 0.5 * (rng.normal(size=shape) + 1j * rng.normal(size=shape))).astype(np.complex64)
+    got = em.fsc(to_device(a), to_device(b)).cpu().numpy()
+    ref = oracle.fsc(a, b, big)
+    np.testing.assert_allclose(got, ref, atol=1e-9)
+    sums = em.shell_sums(to_device(a), to_device(b)).cpu().numpy()
+    assert sums[3].sum() == (oracle.shell_index(big) < big // 2 + 1).sum()
+
+
+def test_gaussian_blob_is_reconstructed():
+    # tests/integration/src/em/functional/test_cpu_reconstruction.cpp:418-503:
+    # one 16^2 Gaussian image (width 2) under 400 uniform rotations, linear
+    # window into 32^3, quotient above 1e-3, irfft3; correlation with the
+    # blob over the central 17^3 box > 0.99.
+    n, big, width = 16, 32, 2.0
+    r = np.arange(n) - n // 2
+    image = np.exp(-(r[None, :] ** 2 + r[:, None] ** 2) / (2 * width ** 2)).astype(np.float32)
+    rng = np.random.default_rng(14)
+    count = 400
+    transforms = synthetic.transforms_of(synthetic.uniform_rotations(count, rng))
+    images = torch.from_numpy(np.repeat(image[None], count, axis=0)).cuda()
+    slices = rexlib_amd.rfft2(images)
+    num = torch.zeros((big, big, big // 2 + 1), dtype=torch.complex64, device="cuda")
+    den = torch.zeros((big, big, big // 2 + 1), dtype=torch.float32, device="cuda")
+    em.insert_slices(num, den, slices, to_device(transforms), torch.ones(n // 2 + 1, device="cuda"),
+                     em.GriddingWindow.linear(), numerator_out=num, denominator_out=den)
+    volume = rexlib_amd.irfft3(em.normalize(num, den, 1e-3)).cpu().numpy().astype(np.float64)
+    c = big // 2
+    z, y, x = np.meshgrid(*(np.arange(big) - c,) * 3, indexing="ij")
+    box = (np.abs(z) <= n // 2) & (np.abs(y) <= n // 2) & (np.abs(x) <= n // 2)
+    blob = np.exp(-(x ** 2 + y ** 2 + z ** 2) / (2 * width ** 2))
+    corr = (volume[box] * blob[box]).sum() / math.sqrt((volume[box] ** 2).sum() * (blob[box] ** 2).sum())
+    assert corr > 0.99
+
+
+@pytest.mark.parametrize("count", [1000])
+def test_plain_cpu_configuration_end_to_end(count):
+    """The 64^2 -> 64^3 configuration: 1000 noisy CTF-modulated projections of
+    50 blobs, trilinear insertion, quotient, irfft3 -- against the reference's
+    float32 kernels (or the double-precision oracle) on the same images."""
+    n = big = 64
+    vol = synthetic.blob_volume(big, 50, 24.0, (1.5, 4.0), seed=0)
+    stack = synthetic.particle_stack(vol, count, n, seed=1, snr=0.1, pixel_size=1.5, device="cuda")
+    images = stack.images
+    transforms = to_device(stack.transforms)
+    ctf = to_device(stack.ctf)
+    radial = torch.ones(n // 2 + 1, device="cuda")
+    num = torch.zeros((big, big, big // 2 + 1), dtype=torch.complex64, device="cuda")
+    den = torch.zeros((big, big, big // 2 + 1), dtype=torch.float32, device="cuda")
+    slices = rexlib_amd.rfft2(images)
+    em.insert_ctf_slices(num, den, slices, transforms, ctf, radial, em.GriddingWindow.linear(), OPTICS, 1.5,
+                         numerator_out=num, denominator_out=den)
+    volume = rexlib_amd.irfft3(em.normalize(num, den, 1e-3))
+    torch.cuda.synchronize()
+
+    host_images = images.cpu().numpy()
+    host_slices = np.fft.rfft2(host_images).astype(np.complex64)
+    acc = np.zeros((big, big, big // 2 + 1), np.complex64)
+    inputs = [acc, acc.real.copy(), host_slices, stack.transforms.astype(np.float32),
+              stack.ctf.astype(np.float32), np.ones(n // 2 + 1, np.float32)]
+    if oracle.ref_available():
+        ref = oracle.ref_insert("ctf", inputs, oracle.Window.linear(), pixel_size=1.5, dtype=np.float32,
+                                threads=8)
+    else:
+        ref = oracle.insert_f64("ctf", inputs, oracle.Window.linear(), pixel_size=1.5)
+    assert oracle.rel_l2(slices.cpu().numpy(), host_slices) <= 1e-5
+    assert oracle.rel_l2(num.cpu().numpy(), ref[0]) <= 1e-4
+    assert oracle.rel_l2(den.cpu().numpy(), ref[1]) <= 1e-4
+    ref_volume = oracle.irfft3(oracle.normalize(ref[0].astype(np.complex128), ref[1].astype(np.float64)), big)
+    assert oracle.rel_l2(volume.cpu().numpy(), ref_volume) <= 1e-4
