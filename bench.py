@@ -1,0 +1,255 @@
+"""Benchmark: images inserted per second, 256^2 images into a 256^3 half-complex grid.
+
+Workload (the plain GPU configuration): a 256^3 volume of 400 Gaussian blobs
+(sigma 2-8 voxels), 50,000 projections of 256 x 256 float32 at uniform
+orientations, CTF defocus 1-3 um with up to 0.1 um astigmatism (300 kV,
+Cs 2.7 mm, Q 0.1, 1.5 A pixels), white noise at SNR 0.1. One step is the whole
+reconstruction: zero the accumulators, then for every launch of 4,096 images
+rfft2 + CTF-weighted trilinear insertion into 256 x 256 x 129 complex64 +
+float32 accumulators, then the weight quotient, irfft3 and gridding
+correction. Inputs are resident in HBM before the timed region.
+
+Printed: one JSON line with the throughput, the roofline of the dominant
+kernel, the CPU baseline (the reference's own insertion kernels, compiled from
+its sources, on the host cores) and the parity of a sample against it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("images inserted/s 256²→256³, plain FP32 vs BF16+batched+learned, "
+          "at rel-L2≤1e-4")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*args):
+    print(*args, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_image(n, big, launch, pixel_bytes=4):
+    """n^2 s_pix + 64 + 24 N^2 (N/2+1) / L: the image, its transform and CTF,
+    and the accumulators read and written once per launch."""
+    return n * n * pixel_bytes + 64 + 24.0 * big * big * (big // 2 + 1) / launch
+
+
+def make_workload(args, torch):
+    from rexlib_amd import synthetic
+    t0 = time.time()
+    vol = synthetic.blob_volume(args.big, args.blobs, args.big * 0.375, (2.0, 8.0), seed=1000)
+    stack = synthetic.particle_stack(vol, args.images, args.n, seed=1001, snr=0.1, pixel_size=1.5,
+                                     defocus=(10000.0, 30000.0), astigmatism=1000.0, device="cuda",
+                                     chunk=512)
+    torch.cuda.synchronize()
+    log(f"generated {args.images} images of {args.n}^2 in {time.time() - t0:.1f} s")
+    return stack
+
+
+def cpu_baseline(args, stack, threads):
+    """The reference's insertion kernels (oracle/_ref) plus pocketfft rfft2 on
+    the host, over a bounded sample of the same images, timed on `threads` cores."""
+    import oracle
+    import scipy.fft
+    if not oracle.ref_available():
+        return None
+    n, big = args.n, args.big
+    h = n // 2 + 1
+    acc = np.zeros((big, big, big // 2 + 1), np.complex64)
+    den = np.zeros((big, big, big // 2 + 1), np.float32)
+    chunk = 256
+    done = 0
+    elapsed = 0.0
+    start = 0
+    images = stack.images
+    # Warm-up chunk, untimed.
+    warm = images[:16].cpu().numpy()
+    inputs = [acc, den, scipy.fft.rfft2(warm, workers=threads).astype(np.complex64),
+              stack.transforms[:16].astype(np.float32), stack.ctf[:16].astype(np.float32), np.ones(h, np.float32)]
+    oracle.ref_insert("ctf", inputs, oracle.Window.linear(), pixel_size=1.5, threads=threads)
+    while elapsed < args.cpu_seconds and start + chunk <= images.shape[0]:
+        host = images[start:start + chunk].cpu().numpy()
+        t0 = time.perf_counter()
+        spectra = scipy.fft.rfft2(host, workers=threads).astype(np.complex64)
+        inputs = [acc, den, spectra, stack.transforms[start:start + chunk].astype(np.float32),
+                  stack.ctf[start:start + chunk].astype(np.float32), np.ones(h, np.float32)]
+        acc, den = oracle.ref_insert("ctf", inputs, oracle.Window.linear(), pixel_size=1.5, threads=threads)
+        elapsed += time.perf_counter() - t0
+        done += chunk
+        start += chunk
+    return {
+        "value": done / elapsed,
+        "unit": "images/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": (f"{done} images of {n}^2 into one {big}^3 grid (rfft2 by scipy.fft/pocketfft, "
+                   f"CTF trilinear insertion by the reference's row kernel, tile-owned over "
+                   f"{threads} threads), {elapsed:.1f} s"),
+    }
+
+
+def parity_sample(args, stack, torch, em, rexlib_amd, count=256):
+    """rel-L2 of the GPU accumulators against the reference's float32 kernels on
+    the first `count` images."""
+    import oracle
+    n, big = args.n, args.big
+    h = n // 2 + 1
+    images = stack.images[:count]
+    num = torch.zeros((big, big, big // 2 + 1), dtype=torch.complex64, device="cuda")
+    den = torch.zeros((big, big, big // 2 + 1), dtype=torch.float32, device="cuda")
+    spectra = rexlib_amd.rfft2(images)
+    em.insert_ctf_slices(num, den, spectra, torch.from_numpy(stack.transforms[:count].astype(np.float32)).cuda(),
+                         torch.from_numpy(stack.ctf[:count].astype(np.float32)).cuda(),
+                         torch.ones(h, device="cuda"), em.GriddingWindow.linear(), em.CtfOptics(300, 2.7, 0.1), 1.5,
+                         numerator_out=num, denominator_out=den)
+    torch.cuda.synchronize()
+    acc = np.zeros((big, big, big // 2 + 1), np.complex64)
+    host_spectra = spectra.cpu().numpy()
+    inputs = [acc, acc.real.copy(), host_spectra, stack.transforms[:count].astype(np.float32),
+              stack.ctf[:count].astype(np.float32), np.ones(h, np.float32)]
+    if oracle.ref_available():
+        ref = oracle.ref_insert("ctf", inputs, oracle.Window.linear(), pixel_size=1.5,
+                                threads=min(16, os.cpu_count() or 1))
+        against = "reference kernels, float32"
+    else:
+        ref = oracle.insert_f64("ctf", inputs, oracle.Window.linear(), pixel_size=1.5)
+        against = "double-precision oracle"
+    return {
+        "images": count,
+        "against": against,
+        "rel_l2_numerator": oracle.rel_l2(num.cpu().numpy(), ref[0]),
+        "rel_l2_denominator": oracle.rel_l2(den.cpu().numpy(), ref[1]),
+    }
+
+
+def main():
+    parser = argparse.ArgumentParser(description=__doc__)
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=3)
+    parser.add_argument("--warmup", type=int, default=1)
+    parser.add_argument("--images", type=int, default=50000)
+    parser.add_argument("--launch", type=int, default=4096)
+    parser.add_argument("--n", type=int, default=256)
+    parser.add_argument("--big", type=int, default=256)
+    parser.add_argument("--blobs", type=int, default=400)
+    parser.add_argument("--cpu-seconds", type=float, default=12.0)
+    parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-parity", action="store_true")
+    args = parser.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path would shard by "
+              "images with one all-reduce of the accumulators; see the design notes).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import rexlib_amd
+    from rexlib_amd import em
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    stack = make_workload(args, torch)
+    n, big, L = args.n, args.big, args.launch
+    h = n // 2 + 1
+    transforms = torch.from_numpy(stack.transforms.astype(np.float32)).cuda()
+    ctf = torch.from_numpy(stack.ctf.astype(np.float32)).cuda()
+    radial = torch.ones(h, device="cuda")
+    num = torch.zeros((big, big, big // 2 + 1), dtype=torch.complex64, device="cuda")
+    den = torch.zeros((big, big, big // 2 + 1), dtype=torch.float32, device="cuda")
+    spectra = torch.empty((L, n, h), dtype=torch.complex64, device="cuda")
+    window = em.GriddingWindow.linear()
+    optics = em.CtfOptics(300.0, 2.7, 0.1)
+    launches = [(s, min(s + L, args.images)) for s in range(0, args.images, L)]
+
+    insert_events = []
+
+    def step(record):
+        num.zero_()
+        den.zero_()
+        for s, e in launches:
+            out = spectra[: e - s]
+            rexlib_amd.rfft2(stack.images[s:e], out=out)
+            if record:
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record()
+            em.insert_ctf_slices(num, den, out, transforms[s:e], ctf[s:e], radial, window, optics, 1.5,
+                                 numerator_out=num, denominator_out=den)
+            if record:
+                b.record()
+                insert_events.append((a, b, e - s))
+        volume = rexlib_amd.irfft3(em.normalize(num, den, 1e-3))
+        return em.gridding_correct(volume, window, out=volume)
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = args.images * args.steps / elapsed
+    log(f"{args.steps} steps of {args.images} images: {elapsed:.3f} s, {value:,.0f} images/s")
+
+    full = [(a.elapsed_time(b), c) for a, b, c in insert_events if c == L]
+    avg_ms = float(np.mean([t for t, _ in full])) if full else float("nan")
+    b_img = algorithmic_bytes_per_image(n, big, L)
+    achieved = b_img * L / (avg_ms * 1e-3) / 1e9 if full else None
+    roofline = {
+        "bound": "hbm",
+        "kernel": "insertion (settle + scatter kernels) per launch of 4096 images",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS if achieved else None,
+        "traffic": None,
+        "launch_ms": avg_ms,
+        "bytes_per_image": b_img,
+        "pipeline_frac": value * b_img / (HBM_PEAK_GBS * 1e9),
+    }
+
+    parity = None if args.no_parity else parity_sample(args, stack, torch, em, rexlib_amd)
+    if parity:
+        log(f"parity: {parity}")
+    threads = min(16, os.cpu_count() or 1)
+    baseline = None if args.no_cpu_baseline else cpu_baseline(args, stack, threads)
+    if baseline:
+        log(f"cpu baseline: {baseline}")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "images/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (blob volume, exact projections, CTF, SNR 0.1)",
+        "config": {
+            "workload": (f"plain FP32: {args.images} images {n}^2 -> {big}^3, launches of {L}, CTF + trilinear, "
+                         "then quotient, irfft3, gridding correction"),
+            "images": args.images, "n": n, "N": big, "launch": L, "window": "linear", "variant": "ctf",
+        },
+        "roofline": roofline,
+        "cpu_baseline": baseline,
+        "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

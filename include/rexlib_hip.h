@@ -93,6 +93,15 @@ void rexlib_hip_version(int32_t *major, int32_t *minor, int32_t *patch);
  * (plugins/cuda/src/hardware/cuda_device_backend.cpp:37-42). */
 int32_t rexlib_hip_device_count(void);
 
+/* Tuning of the LDS brick schedule of the insertions, process-wide: brick
+ * extents along y and z (8 or 16; 16 along x), and the expected plane hits
+ * per task that splits busy bricks (>= 16). 0 keeps the default.
+ * No reference counterpart (the reference's CPU tuning is
+ * slice_insertion_tuning, src/backends/cpu/builders/em/reconstruction/slice_insertion_tuning.cpp:10-13);
+ * set by the autotuner (rexlib_amd/tune.py). */
+void rexlib_hip_set_insertion_tuning(int32_t extent_y, int32_t extent_z, int32_t share);
+void rexlib_hip_get_insertion_tuning(int32_t *extent_y, int32_t *extent_z, int32_t *share);
+
 /* ------------------------------------------------------------------------- */
 /* Slice insertion — the hot path                                            */
 /* ------------------------------------------------------------------------- */

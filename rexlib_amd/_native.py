@@ -55,6 +55,8 @@ EXPORTS = [
     "rexlib_hip_last_error",
     "rexlib_hip_version",
     "rexlib_hip_device_count",
+    "rexlib_hip_set_insertion_tuning",
+    "rexlib_hip_get_insertion_tuning",
     "rexlib_hip_insert_slices",
     "rexlib_hip_insert_weighted_slices",
     "rexlib_hip_insert_ctf_slices",
@@ -89,6 +91,10 @@ def library() -> ctypes.CDLL:
     lib.rexlib_hip_version.restype = None
     lib.rexlib_hip_device_count.restype = ctypes.c_int32
     lib.rexlib_hip_device_count.argtypes = []
+    lib.rexlib_hip_set_insertion_tuning.argtypes = [ctypes.c_int32] * 3
+    lib.rexlib_hip_set_insertion_tuning.restype = None
+    lib.rexlib_hip_get_insertion_tuning.argtypes = [P(ctypes.c_int32)] * 3
+    lib.rexlib_hip_get_insertion_tuning.restype = None
     lib.rexlib_hip_insert_slices.argtypes = [A, A, A, A, A, W, A, A, vp]
     lib.rexlib_hip_insert_weighted_slices.argtypes = [A, A, A, A, A, A, W, A, A, vp]
     lib.rexlib_hip_insert_ctf_slices.argtypes = [A, A, A, A, A, A, W, O, ctypes.c_double, A, A, vp]
@@ -99,7 +105,7 @@ def library() -> ctypes.CDLL:
     lib.rexlib_hip_normalize.argtypes = [A, A, ctypes.c_double, A, vp]
     lib.rexlib_hip_gridding_correct.argtypes = [A, W, A, vp]
     lib.rexlib_hip_shell_sums.argtypes = [A, A, vp, vp]
-    for name in EXPORTS[3:]:
+    for name in EXPORTS[5:]:
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib

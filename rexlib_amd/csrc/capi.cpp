@@ -438,6 +438,29 @@ void rexlib_hip_version(int32_t *major, int32_t *minor, int32_t *patch)
 	}
 }
 
+void rexlib_hip_set_insertion_tuning(int32_t extent_y, int32_t extent_z, int32_t share)
+{
+	rexlib_hip::set_brick_tuning(extent_y, extent_z, share);
+}
+
+void rexlib_hip_get_insertion_tuning(int32_t *extent_y, int32_t *extent_z, int32_t *share)
+{
+	int w = 0, d = 0, s = 0;
+	rexlib_hip::get_brick_tuning(&w, &d, &s);
+	if (extent_y)
+	{
+		*extent_y = w;
+	}
+	if (extent_z)
+	{
+		*extent_z = d;
+	}
+	if (share)
+	{
+		*share = s;
+	}
+}
+
 int32_t rexlib_hip_device_count(void)
 {
 	int count = 0;

@@ -21,7 +21,12 @@
 //     (slice_insertion_functor.inl:105-156, 271-465).
 #include "launch.hpp"
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
+#include <tuple>
+#include <map>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -84,7 +89,23 @@ __global__ void settle_kernel(insertion_params p)
 			// A non-finite frame bounds no rows (src/backends/cpu/kernels/row_interval.cpp).
 			e.last_shell = finite ? last : -1;
 			e.ctf_radial = e.ctf_axial = e.ctf_diagonal = e.ctf_aberration = e.ctf_phase = 0.0f;
-			e.pad = 0;
+			// Two samples of one parity class differ by 2 a X + 2 b Y; when the
+			// smallest eigenvalue of the Gram matrix of (X, Y) exceeds 3/4, that
+			// is at least one voxel along some axis, so no two of them fall in
+			// the same voxel cell.
+			{
+				double xx = 0.0, xy = 0.0, yy = 0.0;
+				for (int a = 0; a < 3; ++a)
+				{
+					xx += static_cast<double>(e.x_step[a]) * e.x_step[a];
+					xy += static_cast<double>(e.x_step[a]) * e.y_step[a];
+					yy += e.y_step[a] * e.y_step[a];
+				}
+				const double trace = xx + yy;
+				const double gap = sqrt(fmax(0.0, (xx - yy) * (xx - yy) + 4.0 * xy * xy));
+				const double smallest = 0.5 * (trace - gap);
+				e.lattice_free = finite && smallest > 0.76 ? 1 : 0;
+			}
 			e.extra_offset = p.extra != nullptr ? offset(slot_extra) : 0;
 			if (p.variant == variant_ctf)
 			{
@@ -491,6 +512,669 @@ void dispatch_scatter(const insertion_params &p, hipStream_t stream)
 	}
 }
 
+// ---------------------------------------------------------------------------
+// Brick schedule: one wavefront owns a brick of the volume in LDS.
+//
+// First, a prepare kernel computes every stored element of every slice once:
+// v * r * (numerator weight) and r * (denominator weight), or zeros when the
+// element is not inserted (row_insertion_kernel.inl:180-279).
+//
+// The brick is a box of signed frequencies (x in [0, N/2], y and z in
+// [-N/2, N/2]). The wavefront tests 64 slices at a time for planes that pass
+// near the brick; for each such slice it bounds the rows that reach the
+// brick and clips each row to the samples that reach it -- bound_rows and
+// clip_row of src/backends/cpu/kernels/row_interval.cpp, restated per brick
+// as the reference restates them per tile (slice_insertion_functor.inl:391-440).
+//
+// The samples are added by plain LDS read-modify-write, not LDS float
+// atomics (which retire at about a third of a lane per clock per CU on
+// gfx950, against eight for a read-modify-write). A lane owns one sample per
+// instruction, and the samples are taken in four parity classes of
+// (k_x mod 2, k_y mod 2): two samples of a class differ by 2 a X + 2 b Y,
+// which is at least one voxel along some axis when the frame (X, Y) is close
+// enough to orthogonal with steps of at least about 0.87 voxel (every rotation
+// with N >= n), so no two lanes of an instruction touch one voxel. Slices
+// whose frames fail that test take LDS atomics instead. The wavefront is the
+// brick's only writer, so there is no barrier anywhere.
+//
+// Every plane passes through the origin, so bricks near it see every slice
+// and bricks far out see few (about 13 / r of them). A task is a brick and a
+// share of the slices: heavy bricks are split, and the heaviest tasks start
+// first. A brick that one task owns is added to the accumulators by plain
+// read-add-write; split bricks, and bricks holding the y or z frequency
+// +-N/2 of an even N (two signed frequencies, one stored element), add with
+// atomics.
+// ---------------------------------------------------------------------------
+
+constexpr int brick_edge = 16; // along x; the extents along y and z are parameters
+// LDS pitches of a brick: odd, so that lanes stepping along any axis fall in
+// different banks.
+constexpr int pitch_y = brick_edge + 1;
+
+__host__ __device__ constexpr int brick_pitch_z(int extent_y)
+{
+	return pitch_y * extent_y + 1;
+}
+
+struct brick_box
+{
+	int lo[3];  // signed (x, y, z) of the first voxel
+	int ext[3]; // voxels along (x, y, z)
+	bool alias_free;
+};
+
+__device__ inline brick_box unpack_brick(int4 b)
+{
+	brick_box box;
+	box.lo[0] = b.x;
+	box.lo[1] = b.y;
+	box.lo[2] = b.z;
+	box.ext[0] = b.w & 0xff;
+	box.ext[1] = (b.w >> 8) & 0xff;
+	box.ext[2] = (b.w >> 16) & 0xff;
+	box.alias_free = ((b.w >> 24) & 1) != 0;
+	return box;
+}
+
+__device__ inline int wave_inclusive_scan(int value)
+{
+	const int lane = threadIdx.x & 63;
+	for (int offset = 1; offset < 64; offset <<= 1)
+	{
+		const int other = __shfl_up(value, offset, 64);
+		if (lane >= offset)
+		{
+			value += other;
+		}
+	}
+	return value;
+}
+
+// Keeps the compiler from moving LDS accesses across it: the taps of other
+// lanes may alias, which it cannot see.
+__device__ inline void lds_order()
+{
+	__asm__ __volatile__("" ::: "memory");
+}
+
+// The range of slices a brick launch covers, per kept accumulator:
+// reduced indices [begin, begin + count).
+struct slice_chunk
+{
+	int64_t begin;
+	int64_t count;
+};
+
+template <int VARIANT>
+__global__ __launch_bounds__(256) void prepare_kernel(insertion_params p, slice_chunk chunk, float4 *prepared)
+{
+	const int64_t elements = static_cast<int64_t>(p.n) * p.h;
+	const int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+	if (idx >= elements)
+	{
+		return;
+	}
+	const int row = static_cast<int>(idx / p.h);
+	const int column = static_cast<int>(idx % p.h);
+	const int64_t slices = p.map.kept_count * chunk.count;
+	for (int64_t j = blockIdx.y; j < slices; j += gridDim.y)
+	{
+		const int64_t kept = j / chunk.count;
+		const int64_t s = kept * p.map.reduced_count + chunk.begin + j % chunk.count;
+		const slice_entry &e = p.entries[s];
+		float2 value = make_float2(0.0f, 0.0f);
+		float weight = 0.0f;
+		if (e.last_shell < 0 || !prepare_sample<VARIANT>(p, e, s, row, column, value, weight))
+		{
+			value = make_float2(0.0f, 0.0f);
+			weight = 0.0f;
+		}
+		prepared[j * elements + idx] = make_float4(value.x, value.y, weight, 0.0f);
+	}
+}
+
+template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
+__device__ inline void add_sample(const insertion_params &p, const brick_box &box, float4 *acc, const float *c,
+	float vr, float vi, float weight)
+{
+	constexpr int pitch_z = brick_pitch_z(EY);
+	float wx[window_taps<PROFILE>::taps];
+	float wy[window_taps<PROFILE>::taps];
+	float wz[window_taps<PROFILE>::taps];
+	const int gx0 = window_taps<PROFILE>::place(p, c[0], wx);
+	const int gy0 = window_taps<PROFILE>::place(p, c[1], wy);
+	const int gz0 = window_taps<PROFILE>::place(p, c[2], wz);
+	const int taps = PROFILE == REXLIB_HIP_KAISER_BESSEL ? p.taps : window_taps<PROFILE>::taps;
+	const int big = p.big;
+	for (int tz = 0; tz < taps; ++tz)
+	{
+		const int lz = gz0 + tz - box.lo[2];
+		const bool in_z = static_cast<unsigned>(lz) < static_cast<unsigned>(box.ext[2]);
+		const int gz = gz0 + tz;
+		const int sz = gz < 0 ? gz + big : gz;
+		for (int ty = 0; ty < taps; ++ty)
+		{
+			const int ly = gy0 + ty - box.lo[1];
+			const bool in_zy = in_z && static_cast<unsigned>(ly) < static_cast<unsigned>(box.ext[1]);
+			const int gy = gy0 + ty;
+			const int sy = gy < 0 ? gy + big : gy;
+			const float wzy = wz[tz] * wy[ty];
+			for (int tx = 0; tx < taps; ++tx)
+			{
+				const int lx = gx0 + tx - box.lo[0];
+				const bool inside = in_zy && static_cast<unsigned>(lx) < static_cast<unsigned>(box.ext[0]);
+				const int gx = gx0 + tx;
+				const float w = wzy * wx[tx];
+				const float2 contribution = volume_phase<ODD>(p, make_float2(vr * w, vi * w), gx, sy, sz);
+				const int idx = lx + pitch_y * ly + pitch_z * lz;
+				if (inside)
+				{
+					if (ATOMIC)
+					{
+						atomicAdd(&acc[idx].x, contribution.x);
+						atomicAdd(&acc[idx].y, contribution.y);
+						atomicAdd(&acc[idx].z, weight * w);
+					}
+					else
+					{
+						float4 v = acc[idx];
+						v.x += contribution.x;
+						v.y += contribution.y;
+						v.z += weight * w;
+						acc[idx] = v;
+					}
+				}
+				lds_order();
+			}
+		}
+	}
+}
+
+// One slice, one sign, into a brick; the whole wavefront takes part.
+template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
+__device__ void insert_slice_sign(const insertion_params &p, const slice_entry &e, const float4 *slice,
+	const brick_box &box, const float *lower, const float *upper, int sign_index, float4 *acc)
+{
+	const int lane = threadIdx.x & 63;
+	const int n = p.n;
+	const int h = p.h;
+	const bool even = n % 2 == 0;
+	const int last_column = h - (even ? 2 : 1);
+	const int highest_row = even ? n / 2 - 1 : n / 2;
+	const float radius = static_cast<float>(e.last_shell) + 0.5f;
+	const int widest = min(last_column, e.last_shell);
+	const int first_row = max(-highest_row, -widest);
+	const int last_row = min(highest_row, widest);
+	const float sign = sign_index == 0 ? 1.0f : -1.0f;
+	const int first = sign_index;
+	float step[3];
+	float y_step[3];
+	for (int a = 0; a < 3; ++a)
+	{
+		step[a] = sign * e.x_step[a];
+		y_step[a] = sign * static_cast<float>(e.y_step[a]);
+	}
+	// bound_rows: the rows k_y whose samples k_x in [first, widest] can reach the box.
+	float low = static_cast<float>(first_row);
+	float high = static_cast<float>(last_row);
+	if (first > widest || first_row > last_row)
+	{
+		return;
+	}
+	for (int a = 0; a < 3; ++a)
+	{
+		const float at_first = static_cast<float>(first) * step[a];
+		const float at_last = static_cast<float>(widest) * step[a];
+		const float lo = lower[a] - fmaxf(at_first, at_last);
+		const float hi = upper[a] - fminf(at_first, at_last);
+		if (y_step[a] == 0.0f)
+		{
+			if (!(lo <= 0.0f && 0.0f <= hi))
+			{
+				return;
+			}
+			continue;
+		}
+		const float l = lo / y_step[a];
+		const float u = hi / y_step[a];
+		low = fmaxf(low, fminf(l, u));
+		high = fminf(high, fmaxf(l, u));
+	}
+	if (low > high)
+	{
+		return;
+	}
+	// The box is already widened past the window, so rounding inwards keeps
+	// every sample that can reach it.
+	const int r0 = max(static_cast<int>(ceilf(low)), first_row);
+	const int r1 = min(static_cast<int>(floorf(high)) + 1, last_row + 1);
+	for (int rc = r0; rc < r1; rc += 64)
+	{
+		// clip_row: the samples of row k_y that can reach the box.
+		const int ky = rc + lane;
+		int begin = 0;
+		int end = 0;
+		if (ky < r1)
+		{
+			const float kyf = static_cast<float>(ky);
+			const float room = radius * radius - kyf * kyf;
+			if (room >= 0.0f)
+			{
+				const int last = min(widest, static_cast<int>(sqrtf(room)));
+				float lo_k = static_cast<float>(first);
+				float hi_k = static_cast<float>(last);
+				bool none = first > last;
+				for (int a = 0; a < 3 && !none; ++a)
+				{
+					const float base = kyf * y_step[a];
+					if (step[a] == 0.0f)
+					{
+						none = !(lower[a] <= base && base <= upper[a]);
+						continue;
+					}
+					const float l = (lower[a] - base) / step[a];
+					const float u = (upper[a] - base) / step[a];
+					lo_k = fmaxf(lo_k, fminf(l, u));
+					hi_k = fminf(hi_k, fmaxf(l, u));
+				}
+				if (!none && lo_k <= hi_k)
+				{
+					begin = max(static_cast<int>(ceilf(lo_k)), first);
+					end = max(begin, min(static_cast<int>(floorf(hi_k)) + 1, last + 1));
+				}
+			}
+		}
+		// Four parity classes; a class holds rows of one parity of k_y and the
+		// k_x of one parity within them.
+		for (int cls = 0; cls < 4; ++cls)
+		{
+			const int py = cls >> 1;
+			const int px = cls & 1;
+			int class_begin = begin + ((begin - px) & 1); // first k_x >= begin of parity px
+			int count = 0;
+			if (((ky - py) & 1) == 0 && class_begin < end)
+			{
+				count = (end - class_begin + 1) / 2;
+			}
+			const int inclusive = wave_inclusive_scan(count);
+			const int start = inclusive - count;
+			const int total = __shfl(inclusive, 63, 64);
+			for (int base = 0; base < total; base += 64)
+			{
+				const int i = base + lane;
+				// The row holding sample i: the last lane whose start is at or below i.
+				int lo_r = 0;
+				int hi_r = 63;
+				for (int step_r = 0; step_r < 6; ++step_r)
+				{
+					const int mid = (lo_r + hi_r + 1) >> 1;
+					const int at = __shfl(start, mid, 64);
+					if (at <= i)
+					{
+						lo_r = mid;
+					}
+					else
+					{
+						hi_r = mid - 1;
+					}
+				}
+				const int row_begin = __shfl(class_begin, lo_r, 64);
+				const int row_start = __shfl(start, lo_r, 64);
+				bool active = i < total;
+				const int sky = rc + lo_r;
+				const int kx = row_begin + 2 * (i - row_start);
+				const int row = sky < 0 ? sky + n : sky;
+				float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+				if (active)
+				{
+					q = slice[row * h + kx];
+					active = !(q.x == 0.0f && q.y == 0.0f && q.z == 0.0f);
+				}
+				if (!active)
+				{
+					// Stay in step with the other lanes; add nothing.
+					q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+				}
+				const float fx = static_cast<float>(kx);
+				const float fy = static_cast<float>(sky);
+				float c[3];
+				for (int a = 0; a < 3; ++a)
+				{
+					c[a] = active ? fy * y_step[a] + fx * step[a] : -1.0e6f;
+				}
+				add_sample<PROFILE, ODD, ATOMIC, EY, EZ>(p, box, acc, c, q.x, sign_index == 0 ? q.y : -q.y, q.z);
+			}
+		}
+	}
+}
+
+template <int PROFILE, bool ODD, int EY, int EZ>
+__global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chunk chunk, const float4 *prepared,
+	const int4 *tasks, int task_count, float reach)
+{
+	constexpr int pitch_z = brick_pitch_z(EY);
+	constexpr int voxels_max = pitch_z * EZ;
+	__shared__ float4 acc[voxels_max];
+
+	const int lane = threadIdx.x;
+	const int big = p.big;
+	const int64_t elements = static_cast<int64_t>(p.n) * p.h;
+	const int64_t kept = blockIdx.y;
+
+	for (int ti = blockIdx.x; ti < task_count; ti += gridDim.x)
+	{
+		const brick_box box = unpack_brick(tasks[2 * ti]);
+		const int4 share = tasks[2 * ti + 1];
+		const int64_t share_begin = chunk.count * share.x / share.y;
+		const int64_t share_end = chunk.count * (share.x + 1) / share.y;
+		const int voxels = box.ext[0] * box.ext[1] * box.ext[2];
+		for (int v = lane; v < voxels_max; v += 64)
+		{
+			acc[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		}
+		lds_order();
+		// The box of positions that can reach the brick, widened beyond the
+		// window so that rounding never drops a sample.
+		float lower[3];
+		float upper[3];
+		float centre[3];
+		float half_extent[3];
+		float half_diagonal2 = 0.0f;
+		for (int a = 0; a < 3; ++a)
+		{
+			lower[a] = static_cast<float>(box.lo[a]) - reach - 0.01f;
+			upper[a] = static_cast<float>(box.lo[a] + box.ext[a] - 1) + reach + 0.01f;
+			centre[a] = static_cast<float>(box.lo[a]) + 0.5f * static_cast<float>(box.ext[a] - 1);
+			half_extent[a] = 0.5f * static_cast<float>(box.ext[a] - 1) + reach + 0.02f;
+			half_diagonal2 += half_extent[a] * half_extent[a];
+		}
+		const float half_diagonal = sqrtf(half_diagonal2);
+		const float centre_norm = sqrtf(centre[0] * centre[0] + centre[1] * centre[1] + centre[2] * centre[2]);
+
+		for (int64_t window = share_begin; window < share_end; window += 64)
+		{
+			// Slices whose plane passes within reach of the brick, one per lane.
+			const int64_t j = window + lane;
+			bool near = false;
+			if (j < share_end)
+			{
+				const slice_entry &e = p.entries[kept * p.map.reduced_count + chunk.begin + j];
+				if (e.last_shell >= 0)
+				{
+					const float x0 = e.x_step[0], x1 = e.x_step[1], x2 = e.x_step[2];
+					const float y0 = static_cast<float>(e.y_step[0]);
+					const float y1 = static_cast<float>(e.y_step[1]);
+					const float y2 = static_cast<float>(e.y_step[2]);
+					const float nx = x1 * y2 - x2 * y1;
+					const float ny = x2 * y0 - x0 * y2;
+					const float nz = x0 * y1 - x1 * y0;
+					// The plane meets the box iff |n . c| is at most the box's
+					// support sum_a |n_a| h_a along the normal n.
+					const float distance = fabsf(nx * centre[0] + ny * centre[1] + nz * centre[2]);
+					const float support = fabsf(nx) * half_extent[0] + fabsf(ny) * half_extent[1] +
+						fabsf(nz) * half_extent[2];
+					const float radius = static_cast<float>(e.last_shell) + 0.5f;
+					const float spread =
+						radius * sqrtf(x0 * x0 + x1 * x1 + x2 * x2 + y0 * y0 + y1 * y1 + y2 * y2);
+					near = distance <= support * 1.001f + 1e-3f &&
+						centre_norm <= spread * 1.001f + half_diagonal + 1e-3f;
+				}
+			}
+			unsigned long long mask = __ballot(near);
+			while (mask != 0)
+			{
+				const int bit = __ffsll(static_cast<long long>(mask)) - 1;
+				mask &= mask - 1;
+				const int64_t js = window + bit;
+				const slice_entry &e = p.entries[kept * p.map.reduced_count + chunk.begin + js];
+				const float4 *slice = prepared + (kept * chunk.count + js) * elements;
+				const bool lattice_free = e.lattice_free != 0;
+				for (int sign_index = 0; sign_index < 2; ++sign_index)
+				{
+					if (lattice_free)
+					{
+						insert_slice_sign<PROFILE, ODD, false, EY, EZ>(p, e, slice, box, lower, upper, sign_index,
+							acc);
+					}
+					else
+					{
+						insert_slice_sign<PROFILE, ODD, true, EY, EZ>(p, e, slice, box, lower, upper, sign_index,
+							acc);
+					}
+				}
+			}
+		}
+		lds_order();
+		// Add the brick to the accumulators of its kept batch element.
+		const bool exclusive = box.alias_free && share.y == 1;
+		const int64_t num_base = kept_offset(p.map, slot_numerator_out, kept);
+		const int64_t den_base = kept_offset(p.map, slot_denominator_out, kept);
+		for (int v = lane; v < voxels; v += 64)
+		{
+			const int lx = v % box.ext[0];
+			const int rest = v / box.ext[0];
+			const int ly = rest % box.ext[1];
+			const int lz = rest / box.ext[1];
+			const int idx = lx + pitch_y * ly + pitch_z * lz;
+			const float4 sum = acc[idx];
+			const float re = sum.x;
+			const float im = sum.y;
+			const float d = sum.z;
+			if (re == 0.0f && im == 0.0f && d == 0.0f)
+			{
+				continue;
+			}
+			const int gx = box.lo[0] + lx;
+			const int gy = box.lo[1] + ly;
+			const int gz = box.lo[2] + lz;
+			const int sy = gy < 0 ? gy + big : gy;
+			const int sz = gz < 0 ? gz + big : gz;
+			float2 *target = p.numerator + num_base + sz * p.num_s0 + sy * p.num_s1 + gx * p.num_s2;
+			float *dtarget = p.denominator + den_base + sz * p.den_s0 + sy * p.den_s1 + gx * p.den_s2;
+			if (exclusive)
+			{
+				float2 t = *target;
+				t.x += re;
+				t.y += im;
+				*target = t;
+				*dtarget += d;
+			}
+			else
+			{
+				atomicAdd(&target->x, re);
+				atomicAdd(&target->y, im);
+				atomicAdd(dtarget, d);
+			}
+		}
+		lds_order();
+	}
+}
+
+// Brick shapes, chosen at run time: extents along y and z (x is 16).
+std::atomic<int> tuned_ey{0};
+std::atomic<int> tuned_ez{0};
+std::atomic<int> tuned_share{0};
+
+int brick_ey_choice()
+{
+	const int v = tuned_ey.load();
+	return v == 16 ? 16 : 8;
+}
+
+int brick_ez_choice()
+{
+	const int v = tuned_ez.load();
+	return v == 16 ? 16 : 8;
+}
+
+template <int PROFILE, int EY, int EZ>
+void launch_brick_shape(const insertion_params &p, slice_chunk chunk, const float4 *prepared, const int4 *tasks,
+	int task_count, float reach, hipStream_t stream)
+{
+	const dim3 grid(static_cast<unsigned>(task_count), static_cast<unsigned>(p.map.kept_count));
+	if (p.big % 2 == 1)
+	{
+		brick_kernel<PROFILE, true, EY, EZ><<<grid, 64, 0, stream>>>(p, chunk, prepared, tasks, task_count, reach);
+	}
+	else
+	{
+		brick_kernel<PROFILE, false, EY, EZ><<<grid, 64, 0, stream>>>(p, chunk, prepared, tasks, task_count, reach);
+	}
+}
+
+template <int PROFILE>
+void launch_brick(const insertion_params &p, slice_chunk chunk, const float4 *prepared, const int4 *tasks,
+	int task_count, float reach, hipStream_t stream)
+{
+	const int ey = brick_ey_choice();
+	const int ez = brick_ez_choice();
+	if (ey == 8 && ez == 8)
+	{
+		launch_brick_shape<PROFILE, 8, 8>(p, chunk, prepared, tasks, task_count, reach, stream);
+	}
+	else if (ey == 8)
+	{
+		launch_brick_shape<PROFILE, 8, 16>(p, chunk, prepared, tasks, task_count, reach, stream);
+	}
+	else if (ez == 8)
+	{
+		launch_brick_shape<PROFILE, 16, 8>(p, chunk, prepared, tasks, task_count, reach, stream);
+	}
+	else
+	{
+		launch_brick_shape<PROFILE, 16, 16>(p, chunk, prepared, tasks, task_count, reach, stream);
+	}
+}
+
+void dispatch_brick(const insertion_params &p, slice_chunk chunk, const float4 *prepared, const int4 *tasks,
+	int task_count, float reach, hipStream_t stream)
+{
+	switch (p.profile)
+	{
+	case REXLIB_HIP_BOX:
+		launch_brick<REXLIB_HIP_BOX>(p, chunk, prepared, tasks, task_count, reach, stream);
+		break;
+	case REXLIB_HIP_TRIANGLE:
+		launch_brick<REXLIB_HIP_TRIANGLE>(p, chunk, prepared, tasks, task_count, reach, stream);
+		break;
+	default:
+		launch_brick<REXLIB_HIP_KAISER_BESSEL>(p, chunk, prepared, tasks, task_count, reach, stream);
+		break;
+	}
+}
+
+void dispatch_prepare(const insertion_params &p, slice_chunk chunk, float4 *prepared, hipStream_t stream)
+{
+	const int64_t elements = static_cast<int64_t>(p.n) * p.h;
+	const dim3 grid(static_cast<unsigned>((elements + 255) / 256),
+		static_cast<unsigned>(std::min<int64_t>(p.map.kept_count * chunk.count, 65535)));
+	switch (p.variant)
+	{
+	case variant_plain:
+		prepare_kernel<variant_plain><<<grid, 256, 0, stream>>>(p, chunk, prepared);
+		break;
+	case variant_weighted:
+		prepare_kernel<variant_weighted><<<grid, 256, 0, stream>>>(p, chunk, prepared);
+		break;
+	default:
+		prepare_kernel<variant_ctf><<<grid, 256, 0, stream>>>(p, chunk, prepared);
+		break;
+	}
+}
+
+// Slices a task aims to hold: each brick is split into shares of about this
+// many expected plane hits.
+int brick_share_target()
+{
+	const int tuned = tuned_share.load();
+	return tuned > 0 ? std::max(16, tuned) : 256;
+}
+
+// The tasks of a volume of extent N receiving `slices` slices per
+// accumulator: (packed box, (share, shares)), heaviest first; cached per
+// device, extent, slice-count bucket, share target and brick shape.
+const int4 *brick_tasks(int big, int64_t slices, int &count)
+{
+	static std::mutex mutex;
+	static std::map<std::tuple<int, int, int64_t, int, int, int>, std::pair<int4 *, int>> cache;
+	int device = 0;
+	REXLIB_HIP_CHECK(hipGetDevice(&device));
+	int64_t bucket = 1;
+	while (bucket < slices)
+	{
+		bucket *= 2;
+	}
+	const int target = brick_share_target();
+	const int ey = brick_ey_choice();
+	const int ez = brick_ez_choice();
+	std::lock_guard<std::mutex> lock(mutex);
+	const auto key = std::make_tuple(device, big, bucket, target, ey, ez);
+	const auto found = cache.find(key);
+	if (found != cache.end())
+	{
+		count = found->second.second;
+		return found->second.first;
+	}
+	const int half = big / 2;
+	// Signed ranges: x in [0, N/2]; y, z in [-(N/2), N/2] (even N: N + 1
+	// values whose two ends are one stored element; odd N: N values).
+	const auto starts = [](int low, int high, int edge)
+	{
+		std::vector<std::pair<int, int>> out;
+		for (int s = low; s <= high; s += edge)
+		{
+			out.emplace_back(s, std::min(edge, high - s + 1));
+		}
+		return out;
+	};
+	struct item
+	{
+		int4 box;
+		int4 share;
+		double load;
+	};
+	std::vector<item> items;
+	for (const auto &z : starts(-half, half, ez))
+	{
+		for (const auto &y : starts(-half, half, ey))
+		{
+			for (const auto &x : starts(0, half, brick_edge))
+			{
+				const bool alias = big % 2 == 0 &&
+					(z.first == -half || z.first + z.second - 1 == half || y.first == -half ||
+						y.first + y.second - 1 == half);
+				const int w = x.second | (y.second << 8) | (z.second << 16) | ((alias ? 0 : 1) << 24);
+				const double cx = x.first + 0.5 * (x.second - 1);
+				const double cy = y.first + 0.5 * (y.second - 1);
+				const double cz = z.first + 0.5 * (z.second - 1);
+				const double r = std::sqrt(cx * cx + cy * cy + cz * cz);
+				// A plane through the origin passes a brick at distance r with
+				// probability about min(1, 13 / r).
+				const double hits = static_cast<double>(bucket) * std::min(1.0, 13.0 / std::max(r, 1.0));
+				const int shares = static_cast<int>(std::min<double>(64.0, std::max(1.0, std::ceil(hits / target))));
+				for (int c = 0; c < shares; ++c)
+				{
+					items.push_back({make_int4(x.first, y.first, z.first, w), make_int4(c, shares, 0, 0),
+						hits / shares});
+				}
+			}
+		}
+	}
+	std::stable_sort(items.begin(), items.end(), [](const item &a, const item &b) { return a.load > b.load; });
+	std::vector<int4> host;
+	for (const auto &i : items)
+	{
+		host.push_back(i.box);
+		host.push_back(i.share);
+	}
+	int4 *device_tasks = nullptr;
+	REXLIB_HIP_CHECK(hipMalloc(&device_tasks, sizeof(int4) * host.size()));
+	REXLIB_HIP_CHECK(hipMemcpy(device_tasks, host.data(), sizeof(int4) * host.size(), hipMemcpyHostToDevice));
+	count = static_cast<int>(items.size());
+	cache[key] = std::make_pair(device_tasks, count);
+	return device_tasks;
+}
+
 struct kb_key
 {
 	int device;
@@ -512,6 +1196,20 @@ struct kb_hash
 };
 
 } // anonymous namespace
+
+void set_brick_tuning(int extent_y, int extent_z, int share)
+{
+	tuned_ey.store(extent_y);
+	tuned_ez.store(extent_z);
+	tuned_share.store(share);
+}
+
+void get_brick_tuning(int *extent_y, int *extent_z, int *share)
+{
+	*extent_y = brick_ey_choice();
+	*extent_z = brick_ez_choice();
+	*share = brick_share_target();
+}
 
 // Host entry of every insertion: fills the workspace and launches.
 void run_insertion(insertion_params p, int schedule, const float2 *num_in, const int64_t *num_in_strides,
@@ -564,18 +1262,44 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 	settle_kernel<<<static_cast<unsigned>(std::min<int64_t>(slices, 1 << 20)), 128, 0, stream>>>(p);
 	REXLIB_HIP_CHECK(hipGetLastError());
 
-	(void)schedule;
-	switch (p.variant)
+	// The brick schedule needs the window to span at most a brick; wider
+	// Kaiser-Bessel windows take the scatter schedule.
+	const bool brick = schedule != schedule_scatter && p.taps <= 4 && p.map.kept_count <= 65535;
+	if (brick)
 	{
-	case variant_plain:
-		dispatch_scatter<variant_plain>(p, stream);
-		break;
-	case variant_weighted:
-		dispatch_scatter<variant_weighted>(p, stream);
-		break;
-	default:
-		dispatch_scatter<variant_ctf>(p, stream);
-		break;
+		// Prepared samples take 16 bytes per stored element; the slices of
+		// one call go through in chunks that keep them within 4 GiB.
+		const int64_t per_slice = static_cast<int64_t>(p.n) * p.h * static_cast<int64_t>(sizeof(float4)) *
+			p.map.kept_count;
+		const int64_t cap = int64_t(4) << 30;
+		const int64_t chunk_count = std::max<int64_t>(1, std::min<int64_t>(p.map.reduced_count, cap / per_slice));
+		float4 *prepared = static_cast<float4 *>(workspace_area_bytes(area_prepared,
+			static_cast<std::size_t>(per_slice * chunk_count), stream));
+		int count = 0;
+		const int4 *tasks = brick_tasks(p.big, chunk_count, count);
+		const float reach = p.profile == REXLIB_HIP_BOX ? 0.5f : (p.profile == REXLIB_HIP_TRIANGLE ? 1.0f :
+			p.kb_half_width);
+		for (int64_t begin = 0; begin < p.map.reduced_count; begin += chunk_count)
+		{
+			const slice_chunk chunk{begin, std::min(chunk_count, p.map.reduced_count - begin)};
+			dispatch_prepare(p, chunk, prepared, stream);
+			dispatch_brick(p, chunk, prepared, tasks, count, reach, stream);
+		}
+	}
+	else
+	{
+		switch (p.variant)
+		{
+		case variant_plain:
+			dispatch_scatter<variant_plain>(p, stream);
+			break;
+		case variant_weighted:
+			dispatch_scatter<variant_weighted>(p, stream);
+			break;
+		default:
+			dispatch_scatter<variant_ctf>(p, stream);
+			break;
+		}
 	}
 	REXLIB_HIP_CHECK(hipGetLastError());
 }

@@ -27,7 +27,7 @@ struct alignas(16) slice_entry
 	float ctf_diagonal;
 	float ctf_aberration; // (pi / 2) Cs L^3 s^4 = aberration |k|^4
 	float ctf_phase;      // phase shift + asin(Q)
-	int32_t pad;
+	int32_t lattice_free; // 1: samples two steps apart on both axes never share a voxel cell
 	int64_t slice_offset;  // elements of the complex slices
 	int64_t extra_offset;  // elements of the weights (weighted) or CTF parameters
 	int64_t radial_offset;

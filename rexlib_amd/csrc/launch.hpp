@@ -13,7 +13,8 @@ enum workspace_area : int
 	area_insertion = 0,
 	area_spectra = 1,
 	area_fft = 2,
-	area_count = 3
+	area_prepared = 3,
+	area_count = 4
 };
 
 void *workspace_area_bytes(int area, std::size_t bytes, hipStream_t stream);
@@ -28,6 +29,9 @@ enum insertion_schedule : int
 
 void run_insertion(insertion_params p, int schedule, const float2 *num_in, const int64_t *num_in_strides,
 	const float *den_in, const int64_t *den_in_strides, bool in_place, double shape, hipStream_t stream);
+
+void set_brick_tuning(int extent_y, int extent_z, int share);
+void get_brick_tuning(int *extent_y, int *extent_z, int *share);
 
 void launch_rfft_rows(int kind, const void *in, const line_set &in_rows, int m, const float *scales,
 	int64_t scale_stride, float2 *out, const line_set &out_rows, int64_t n, hipStream_t stream);
