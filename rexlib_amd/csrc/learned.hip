@@ -1,0 +1,603 @@
+// Density compensation of the accumulators: the iterative estimate and a
+// small learned predictor of it.
+//
+// The reference has no density compensation (its reconstruction divides the
+// numerator by the denominator, tests/integration/src/em/functional/test_cpu_reconstruction.cpp:457-464).
+// Here the compensation is a weight W per voxel of the half spectrum that
+// flattens the gridded sampling density:
+//   - iterative (the reference estimate and the fallback): starting from
+//     W = 1, each pass convolves W * den with K3 = t x t x t, t = (1/6, 2/3,
+//     1/6) -- the autocorrelation of the trilinear window at integer lags --
+//     and divides: W <- W / max(K3 * (W den), eps);
+//   - learned: a multilayer perceptron 10 -> 64 -> 64 -> 64 -> 1 (ReLU)
+//     maps per-voxel features to log W. Its weights are float16 and its
+//     products run on v_mfma_f32_16x16x16_f16 with float32 accumulation, in
+//     the forward pass and in the backward pass that trains it.
+//
+// Features of a voxel g: |g| / (N/2); log(1 + den(g)); for each of the 8
+// octants (sx, sy, sz) in {-1, 1}^3, log(1 + the mean of den over the 7
+// neighbours g + (a sx, b sy, c sz), (a, b, c) in {0, 1}^3 \ 0) -- an 8-bin
+// histogram of where the sampling around g lies.
+//
+// The half spectrum is Hermitian: the full-spectrum element (x, y, z) with x
+// beyond the stored half is the element (N - x, -y, -z), and a real field
+// such as den takes the same value there.
+#include "common.hpp"
+
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+
+namespace rexlib_hip
+{
+
+namespace
+{
+
+// The stored index of full-spectrum frequency (x, y, z), each taken modulo N.
+__device__ inline int64_t hermitian_index(int x, int y, int z, int big, int half)
+{
+	x = ((x % big) + big) % big;
+	y = ((y % big) + big) % big;
+	z = ((z % big) + big) % big;
+	if (x >= half)
+	{
+		x = big - x;
+		y = (big - y) % big;
+		z = (big - z) % big;
+	}
+	return (static_cast<int64_t>(z) * big + y) * half + x;
+}
+
+__global__ void compensation_pass_kernel(const float *den, const float *w_in, float *w_out, int big, float floor_value)
+{
+	const int half = big / 2 + 1;
+	const int64_t count = static_cast<int64_t>(big) * big * half;
+	const float t[3] = {1.0f / 6.0f, 2.0f / 3.0f, 1.0f / 6.0f};
+	for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < count;
+		i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+	{
+		const int x = static_cast<int>(i % half);
+		const int y = static_cast<int>((i / half) % big);
+		const int z = static_cast<int>(i / (static_cast<int64_t>(half) * big));
+		float c = 0.0f;
+		for (int dz = -1; dz <= 1; ++dz)
+		{
+			for (int dy = -1; dy <= 1; ++dy)
+			{
+				for (int dx = -1; dx <= 1; ++dx)
+				{
+					const int64_t j = hermitian_index(x + dx, y + dy, z + dz, big, half);
+					c += t[dz + 1] * t[dy + 1] * t[dx + 1] * w_in[j] * den[j];
+				}
+			}
+		}
+		const float w = w_in[i];
+		w_out[i] = den[i] > 0.0f ? w / fmaxf(c, floor_value) : 0.0f;
+	}
+}
+
+__global__ void fill_kernel(float *out, int64_t count, float value)
+{
+	for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < count;
+		i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+	{
+		out[i] = value;
+	}
+}
+
+// Features of every voxel, as 16 float16 per voxel (the last 6 zero).
+__global__ void features_kernel(const float *den, __half *features, int big)
+{
+	const int half = big / 2 + 1;
+	const int64_t count = static_cast<int64_t>(big) * big * half;
+	for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < count;
+		i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+	{
+		const int x = static_cast<int>(i % half);
+		const int sy = static_cast<int>((i / half) % big);
+		const int sz = static_cast<int>(i / (static_cast<int64_t>(half) * big));
+		const int y = sy <= big / 2 ? sy : sy - big;
+		const int z = sz <= big / 2 ? sz : sz - big;
+		float f[16];
+		f[0] = sqrtf(static_cast<float>(x * x + y * y + z * z)) / (0.5f * static_cast<float>(big));
+		f[1] = log1pf(fmaxf(den[i], 0.0f));
+		int bin = 0;
+		for (int oz = -1; oz <= 1; oz += 2)
+		{
+			for (int oy = -1; oy <= 1; oy += 2)
+			{
+				for (int ox = -1; ox <= 1; ox += 2)
+				{
+					float sum = 0.0f;
+					for (int c = 1; c < 8; ++c)
+					{
+						const int dx = (c & 1) ? ox : 0;
+						const int dy = (c & 2) ? oy : 0;
+						const int dz = (c & 4) ? oz : 0;
+						sum += fmaxf(den[hermitian_index(x + dx, y + dy, z + dz, big, half)], 0.0f);
+					}
+					f[2 + bin] = log1pf(sum / 7.0f);
+					++bin;
+				}
+			}
+		}
+		for (int k = 10; k < 16; ++k)
+		{
+			f[k] = 0.0f;
+		}
+		for (int k = 0; k < 16; ++k)
+		{
+			features[i * 16 + k] = __float2half(f[k]);
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
+// The perceptron on MFMA
+// ---------------------------------------------------------------------------
+
+// Each wavefront works on its own tile; LDS operations of one wavefront
+// retire in order, so only the compiler must be kept from moving LDS
+// accesses across these points (no workgroup barrier: the wavefronts of a
+// workgroup run different numbers of tiles).
+__device__ inline void wave_order()
+{
+	__asm__ __volatile__("" ::: "memory");
+}
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+constexpr int hidden = 64;
+constexpr int inputs = 16; // 10 features, padded
+
+// Parameter layout (float16 copy and float32 master share it), row-major
+// W[k][n]: W1 16x64, W2 64x64, W3 64x64, W4 64x16 (column 0 used), then the
+// biases b1, b2, b3 (64 each) and b4 (16, element 0 used).
+constexpr int off_w1 = 0;
+constexpr int off_w2 = off_w1 + inputs * hidden;
+constexpr int off_w3 = off_w2 + hidden * hidden;
+constexpr int off_w4 = off_w3 + hidden * hidden;
+constexpr int off_b1 = off_w4 + hidden * 16;
+constexpr int off_b2 = off_b1 + hidden;
+constexpr int off_b3 = off_b2 + hidden;
+constexpr int off_b4 = off_b3 + hidden;
+constexpr int parameter_count = off_b4 + 16;
+
+// D (16x16, float) += A (16x16, fp16) B (16x16, fp16) on one wavefront. Lane
+// l holds A[l % 16][4 (l / 16) + j], B[4 (l / 16) + j][l % 16] and
+// D[4 (l / 16) + j][l % 16], j = 0..3.
+__device__ inline float4_t mfma16(half4_t a, half4_t b, float4_t c)
+{
+	return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// Load the B operand tile (rows k0.., columns n0..) of a row-major K x N
+// float16 matrix in LDS with row length `cols`.
+__device__ inline half4_t load_b(const _Float16 *w, int cols, int k0, int n0, int lane)
+{
+	half4_t b;
+	const int k = k0 + 4 * (lane >> 4);
+	const int n = n0 + (lane & 15);
+	for (int j = 0; j < 4; ++j)
+	{
+		b[j] = w[(k + j) * cols + n];
+	}
+	return b;
+}
+
+// Load the A operand tile (rows m0.., columns k0..) of a row-major M x K
+// float16 matrix in LDS with row length `cols`.
+__device__ inline half4_t load_a(const _Float16 *m, int cols, int m0, int k0, int lane)
+{
+	half4_t a;
+	const int row = m0 + (lane & 15);
+	const int k = k0 + 4 * (lane >> 4);
+	for (int j = 0; j < 4; ++j)
+	{
+		a[j] = m[row * cols + k + j];
+	}
+	return a;
+}
+
+// A operand of the transpose of a row-major K x M matrix: A[m][k] = T[k][m].
+__device__ inline half4_t load_a_transposed(const _Float16 *t, int cols, int m0, int k0, int lane)
+{
+	half4_t a;
+	const int m = m0 + (lane & 15);
+	const int k = k0 + 4 * (lane >> 4);
+	for (int j = 0; j < 4; ++j)
+	{
+		a[j] = t[(k + j) * cols + m];
+	}
+	return a;
+}
+
+// B operand of the transpose of a row-major N x K matrix: B[k][n] = T[n][k].
+__device__ inline half4_t load_b_transposed(const _Float16 *t, int cols, int k0, int n0, int lane)
+{
+	half4_t b;
+	const int k = k0 + 4 * (lane >> 4);
+	const int n = n0 + (lane & 15);
+	for (int j = 0; j < 4; ++j)
+	{
+		b[j] = t[n * cols + k + j];
+	}
+	return b;
+}
+
+// One dense layer on a 16-row tile: out (16 x 64) = act(in (16 x K) W (K x 64) + b).
+template <int K, bool RELU>
+__device__ inline void dense_tile(const _Float16 *in, const _Float16 *w, const float *b, _Float16 *out, int lane)
+{
+	for (int nt = 0; nt < hidden / 16; ++nt)
+	{
+		float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+		for (int kt = 0; kt < K / 16; ++kt)
+		{
+			acc = mfma16(load_a(in, K, 0, kt * 16, lane), load_b(w, hidden, kt * 16, nt * 16, lane), acc);
+		}
+		const int col = nt * 16 + (lane & 15);
+		for (int j = 0; j < 4; ++j)
+		{
+			const int row = 4 * (lane >> 4) + j;
+			float v = acc[j] + b[col];
+			if (RELU)
+			{
+				v = fmaxf(v, 0.0f);
+			}
+			out[row * hidden + col] = static_cast<_Float16>(v);
+		}
+	}
+}
+
+// Stage the float16 parameters and float32 biases in LDS.
+struct mlp_lds
+{
+	_Float16 w[off_b1];
+	float b[parameter_count - off_b1];
+};
+
+__device__ inline void stage_parameters(mlp_lds &s, const _Float16 *params_h, const float *params_f)
+{
+	for (int i = threadIdx.x; i < off_b1; i += blockDim.x)
+	{
+		s.w[i] = params_h[i];
+	}
+	for (int i = threadIdx.x; i < parameter_count - off_b1; i += blockDim.x)
+	{
+		s.b[i] = params_f[off_b1 + i];
+	}
+	__syncthreads();
+}
+
+// Per-wave activations: the input tile and the three hidden layers.
+struct tile_lds
+{
+	_Float16 x[16 * inputs];
+	_Float16 h1[16 * hidden];
+	_Float16 h2[16 * hidden];
+	_Float16 h3[16 * hidden];
+	float g[16 * hidden]; // gradient of one layer's output
+	_Float16 gh[16 * hidden]; // gradients as float16 MFMA operands, two in turn
+	_Float16 gh2[16 * hidden];
+};
+
+constexpr int mlp_waves = 4;
+
+// out[i] = MLP(features[i]) for i < count.
+__global__ __launch_bounds__(64 * mlp_waves) void mlp_forward_kernel(const __half *features, const _Float16 *params_h,
+	const float *params_f, float *out, int64_t count)
+{
+	__shared__ mlp_lds s;
+	__shared__ tile_lds tiles[mlp_waves];
+	stage_parameters(s, params_h, params_f);
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	tile_lds &t = tiles[wave];
+	const int64_t tiles_total = (count + 15) / 16;
+	for (int64_t tile = blockIdx.x * static_cast<int64_t>(mlp_waves) + wave; tile < tiles_total;
+		tile += static_cast<int64_t>(gridDim.x) * mlp_waves)
+	{
+		const int64_t row0 = tile * 16;
+		for (int e = lane; e < 16 * inputs; e += 64)
+		{
+			const int64_t r = row0 + e / inputs;
+			t.x[e] = r < count ? static_cast<_Float16>(__half2float(features[r * inputs + e % inputs])) : _Float16(0);
+		}
+		wave_order();
+		dense_tile<inputs, true>(t.x, s.w + off_w1, s.b + (off_b1 - off_b1), t.h1, lane);
+		wave_order();
+		dense_tile<hidden, true>(t.h1, s.w + off_w2, s.b + (off_b2 - off_b1), t.h2, lane);
+		wave_order();
+		dense_tile<hidden, true>(t.h2, s.w + off_w3, s.b + (off_b3 - off_b1), t.h3, lane);
+		wave_order();
+		float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+		for (int kt = 0; kt < hidden / 16; ++kt)
+		{
+			acc = mfma16(load_a(t.h3, hidden, 0, kt * 16, lane), load_b(s.w + off_w4, 16, kt * 16, 0, lane), acc);
+		}
+		if ((lane & 15) == 0)
+		{
+			for (int j = 0; j < 4; ++j)
+			{
+				const int64_t r = row0 + 4 * (lane >> 4) + j;
+				if (r < count)
+				{
+					out[r] = acc[j] + s.b[off_b4 - off_b1];
+				}
+			}
+		}
+		wave_order();
+	}
+}
+
+// Add the gradient of a weight matrix: dW (K x 64) += act_in^T (K x 16) dY (16 x 64), for one tile.
+template <int K>
+__device__ inline void accumulate_weight_gradient(const _Float16 *act_in, int in_cols, const _Float16 *dy,
+	float *grad, float scale, int lane)
+{
+	for (int mt = 0; mt < K / 16; ++mt)
+	{
+		for (int nt = 0; nt < hidden / 16; ++nt)
+		{
+			float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+			acc = mfma16(load_a_transposed(act_in, in_cols, mt * 16, 0, lane), load_b(dy, hidden, 0, nt * 16, lane),
+				acc);
+			const int col = nt * 16 + (lane & 15);
+			for (int j = 0; j < 4; ++j)
+			{
+				const int row = mt * 16 + 4 * (lane >> 4) + j;
+				atomicAdd(&grad[row * hidden + col], acc[j] * scale);
+			}
+		}
+	}
+}
+
+// dX (16 x K) = dY (16 x 64) W^T (64 x K) masked by act_x > 0; written to t.g and t.gh.
+template <int K>
+__device__ inline void backward_input(const _Float16 *dy, const _Float16 *w, const _Float16 *act_x, float *g,
+	_Float16 *gh, int lane)
+{
+	for (int nt = 0; nt < K / 16; ++nt)
+	{
+		float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+		for (int kt = 0; kt < hidden / 16; ++kt)
+		{
+			acc = mfma16(load_a(dy, hidden, 0, kt * 16, lane), load_b_transposed(w, hidden, kt * 16, nt * 16, lane),
+				acc);
+		}
+		const int col = nt * 16 + (lane & 15);
+		for (int j = 0; j < 4; ++j)
+		{
+			const int row = 4 * (lane >> 4) + j;
+			const float v = static_cast<float>(act_x[row * K + col]) > 0.0f ? acc[j] : 0.0f;
+			g[row * hidden + col] = v;
+			gh[row * hidden + col] = static_cast<_Float16>(v);
+		}
+	}
+}
+
+// One pass of forward and backward over `count` samples: the mean squared
+// error of MLP(features) against `target`, scaled by `loss_scale` for the
+// float16 gradients; grads (float32, parameter layout) and loss[0] accumulate.
+__global__ __launch_bounds__(64 * mlp_waves) void mlp_backward_kernel(const __half *features, const float *target,
+	const _Float16 *params_h, const float *params_f, float *grads, float *loss, int64_t count, float loss_scale)
+{
+	__shared__ mlp_lds s;
+	__shared__ tile_lds tiles[mlp_waves];
+	stage_parameters(s, params_h, params_f);
+	const int lane = threadIdx.x & 63;
+	const int wave = threadIdx.x >> 6;
+	tile_lds &t = tiles[wave];
+	const int64_t tiles_total = (count + 15) / 16;
+	const float inverse = 1.0f / static_cast<float>(count);
+	for (int64_t tile = blockIdx.x * static_cast<int64_t>(mlp_waves) + wave; tile < tiles_total;
+		tile += static_cast<int64_t>(gridDim.x) * mlp_waves)
+	{
+		const int64_t row0 = tile * 16;
+		for (int e = lane; e < 16 * inputs; e += 64)
+		{
+			const int64_t r = row0 + e / inputs;
+			t.x[e] = r < count ? static_cast<_Float16>(__half2float(features[r * inputs + e % inputs])) : _Float16(0);
+		}
+		wave_order();
+		dense_tile<inputs, true>(t.x, s.w + off_w1, s.b, t.h1, lane);
+		wave_order();
+		dense_tile<hidden, true>(t.h1, s.w + off_w2, s.b + (off_b2 - off_b1), t.h2, lane);
+		wave_order();
+		dense_tile<hidden, true>(t.h2, s.w + off_w3, s.b + (off_b3 - off_b1), t.h3, lane);
+		wave_order();
+		float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+		for (int kt = 0; kt < hidden / 16; ++kt)
+		{
+			acc = mfma16(load_a(t.h3, hidden, 0, kt * 16, lane), load_b(s.w + off_w4, 16, kt * 16, 0, lane), acc);
+		}
+		// dOut = 2 (out - target) / count, scaled; rows beyond count carry 0.
+		// Lanes with column 0 hold the outputs; share them through t.g.
+		if ((lane & 15) == 0)
+		{
+			for (int j = 0; j < 4; ++j)
+			{
+				const int row = 4 * (lane >> 4) + j;
+				const int64_t r = row0 + row;
+				float d = 0.0f;
+				if (r < count)
+				{
+					const float out = acc[j] + s.b[off_b4 - off_b1];
+					const float diff = out - target[r];
+					atomicAdd(loss, diff * diff * inverse);
+					d = 2.0f * diff * inverse * loss_scale;
+				}
+				t.g[row] = d;
+			}
+		}
+		wave_order();
+		// Layer 4: dW4[c][0] += sum_r h3[r][c] dOut[r]; db4 += sum_r dOut[r];
+		// dH3[r][c] = dOut[r] W4[c][0] (h3 > 0).
+		{
+			const int c = lane;
+			float sum = 0.0f;
+			for (int r = 0; r < 16; ++r)
+			{
+				sum += static_cast<float>(t.h3[r * hidden + c]) * t.g[r];
+			}
+			atomicAdd(&grads[off_w4 + c * 16], sum);
+			if (lane == 0)
+			{
+				float db = 0.0f;
+				for (int r = 0; r < 16; ++r)
+				{
+					db += t.g[r];
+				}
+				atomicAdd(&grads[off_b4], db);
+			}
+			const float w4 = static_cast<float>(s.w[off_w4 + c * 16]);
+			float dout[16];
+			for (int r = 0; r < 16; ++r)
+			{
+				dout[r] = t.g[r];
+			}
+			wave_order();
+			for (int r = 0; r < 16; ++r)
+			{
+				const float v = static_cast<float>(t.h3[r * hidden + c]) > 0.0f ? dout[r] * w4 : 0.0f;
+				t.g[r * hidden + c] = v;
+				t.gh[r * hidden + c] = static_cast<_Float16>(v);
+			}
+		}
+		wave_order();
+		// Layer 3: dW3 += h2^T dH3; db3 += column sums; dH2 = dH3 W3^T (h2 > 0).
+		accumulate_weight_gradient<hidden>(t.h2, hidden, t.gh, grads + off_w3, 1.0f, lane);
+		{
+			float db = 0.0f;
+			for (int r = 0; r < 16; ++r)
+			{
+				db += t.g[r * hidden + lane];
+			}
+			atomicAdd(&grads[off_b3 + lane], db);
+		}
+		wave_order();
+		backward_input<hidden>(t.gh, s.w + off_w3, t.h2, t.g, t.gh2, lane);
+		wave_order();
+		// Layer 2.
+		accumulate_weight_gradient<hidden>(t.h1, hidden, t.gh2, grads + off_w2, 1.0f, lane);
+		{
+			float db = 0.0f;
+			for (int r = 0; r < 16; ++r)
+			{
+				db += t.g[r * hidden + lane];
+			}
+			atomicAdd(&grads[off_b2 + lane], db);
+		}
+		wave_order();
+		backward_input<hidden>(t.gh2, s.w + off_w2, t.h1, t.g, t.gh, lane);
+		wave_order();
+		// Layer 1: dW1 += x^T dH1; db1 += column sums.
+		accumulate_weight_gradient<inputs>(t.x, inputs, t.gh, grads + off_w1, 1.0f, lane);
+		{
+			float db = 0.0f;
+			for (int r = 0; r < 16; ++r)
+			{
+				db += t.g[r * hidden + lane];
+			}
+			atomicAdd(&grads[off_b1 + lane], db);
+		}
+		wave_order();
+	}
+}
+
+// Adam on the float32 master parameters; refreshes the float16 copy. The
+// gradients arrive multiplied by `loss_scale` (except the output layer's,
+// which arrive multiplied too) and are cleared for the next step.
+__global__ void adam_kernel(float *params, _Float16 *params_h, float *grads, float *m, float *v, int count, float lr,
+	float beta1, float beta2, float eps, float correction1, float correction2, float loss_scale)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= count)
+	{
+		return;
+	}
+	const float g = grads[i] / loss_scale;
+	grads[i] = 0.0f;
+	const float mi = beta1 * m[i] + (1.0f - beta1) * g;
+	const float vi = beta2 * v[i] + (1.0f - beta2) * g * g;
+	m[i] = mi;
+	v[i] = vi;
+	const float update = lr * (mi / correction1) / (sqrtf(vi / correction2) + eps);
+	const float p = params[i] - update;
+	params[i] = p;
+	params_h[i] = static_cast<_Float16>(p);
+}
+
+unsigned blocks_of(int64_t count, int per)
+{
+	return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((count + per - 1) / per, 16384)));
+}
+
+} // anonymous namespace
+
+int mlp_parameter_count()
+{
+	return parameter_count;
+}
+
+void launch_compensation(const float *den, float *w, float *scratch, int big, int passes, float floor_value,
+	hipStream_t stream)
+{
+	const int64_t count = static_cast<int64_t>(big) * big * (big / 2 + 1);
+	// Ping-pong between w and scratch, starting so that the last pass
+	// writes w.
+	float *a = passes % 2 == 0 ? w : scratch;
+	float *b = passes % 2 == 0 ? scratch : w;
+	fill_kernel<<<blocks_of(count, 256), 256, 0, stream>>>(a, count, 1.0f);
+	for (int p = 0; p < passes; ++p)
+	{
+		compensation_pass_kernel<<<blocks_of(count, 256), 256, 0, stream>>>(den, a, b, big, floor_value);
+		float *t = a;
+		a = b;
+		b = t;
+	}
+	if (passes == 0)
+	{
+		fill_kernel<<<blocks_of(count, 256), 256, 0, stream>>>(w, count, 1.0f);
+	}
+	REXLIB_HIP_CHECK(hipGetLastError());
+}
+
+void launch_features(const float *den, void *features, int big, hipStream_t stream)
+{
+	const int64_t count = static_cast<int64_t>(big) * big * (big / 2 + 1);
+	features_kernel<<<blocks_of(count, 256), 256, 0, stream>>>(den, static_cast<__half *>(features), big);
+	REXLIB_HIP_CHECK(hipGetLastError());
+}
+
+void launch_mlp_forward(const void *features, const void *params_h, const float *params_f, float *out, int64_t count,
+	hipStream_t stream)
+{
+	mlp_forward_kernel<<<blocks_of((count + 15) / 16, mlp_waves), 64 * mlp_waves, 0, stream>>>(
+		static_cast<const __half *>(features), static_cast<const _Float16 *>(params_h), params_f, out, count);
+	REXLIB_HIP_CHECK(hipGetLastError());
+}
+
+void launch_mlp_backward(const void *features, const float *target, const void *params_h, const float *params_f,
+	float *grads, float *loss, int64_t count, float loss_scale, hipStream_t stream)
+{
+	mlp_backward_kernel<<<blocks_of((count + 15) / 16, mlp_waves), 64 * mlp_waves, 0, stream>>>(
+		static_cast<const __half *>(features), target, static_cast<const _Float16 *>(params_h), params_f, grads, loss,
+		count, loss_scale);
+	REXLIB_HIP_CHECK(hipGetLastError());
+}
+
+void launch_adam(float *params, void *params_h, float *grads, float *m, float *v, float lr, float beta1, float beta2,
+	float eps, int step, float loss_scale, hipStream_t stream)
+{
+	const float c1 = 1.0f - std::pow(beta1, static_cast<float>(step));
+	const float c2 = 1.0f - std::pow(beta2, static_cast<float>(step));
+	adam_kernel<<<(parameter_count + 255) / 256, 256, 0, stream>>>(params, static_cast<_Float16 *>(params_h), grads, m,
+		v, parameter_count, lr, beta1, beta2, eps, c1, c2, loss_scale);
+	REXLIB_HIP_CHECK(hipGetLastError());
+}
+
+} // namespace rexlib_hip
