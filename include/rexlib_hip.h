@@ -244,8 +244,51 @@ int rexlib_hip_gridding_correct(const rexlib_hip_array *volume, const rexlib_hip
 int rexlib_hip_shell_sums(const rexlib_hip_array *a, const rexlib_hip_array *b, double *sums,
 	void *stream);
 
+/* ------------------------------------------------------------------------- */
+/* Density compensation: the iterative estimate and its learned predictor.   */
+/* No reference counterpart (the reference divides the numerator by the      */
+/* denominator in a test, test_cpu_reconstruction.cpp:457-464).              */
+/* ------------------------------------------------------------------------- */
+
+/* weights (N, N, N/2+1) float32 = the iterative density compensation of the
+ * denominator: from W = 1, `passes` times W <- W / max(K3 * (W den), floor),
+ * K3 the trilinear window's autocorrelation (1/6, 2/3, 1/6) per axis;
+ * W = 0 where den <= 0. One contiguous grid. */
+int rexlib_hip_density_compensation(const rexlib_hip_array *denominator, int32_t passes, double floor_value,
+	const rexlib_hip_array *weights, void *stream);
+
+/* features (N, N, N/2+1, 16) float16: per voxel |g|/(N/2), log(1+den), and
+ * the 8 octant means of den around it (log(1+.)), then 6 zeros. */
+int rexlib_hip_compensation_features(const rexlib_hip_array *denominator, const rexlib_hip_array *features,
+	void *stream);
+
+/* The perceptron 10 -> 64 -> 64 -> 64 -> 1 (ReLU) on v_mfma_f32_16x16x16_f16:
+ * parameters are rexlib_hip_mlp_parameter_count() elements, row-major
+ * W1 (16x64), W2 (64x64), W3 (64x64), W4 (64x16, column 0), b1, b2, b3 (64)
+ * and b4 (16, element 0), as float16 (the weights used) and float32 (the
+ * master copy, whose biases are used). */
+int32_t rexlib_hip_mlp_parameter_count(void);
+
+/* out (rows) float32 = MLP(features (rows, 16) float16). */
+int rexlib_hip_mlp_forward(const rexlib_hip_array *features, const rexlib_hip_array *parameters_f16,
+	const rexlib_hip_array *parameters_f32, const rexlib_hip_array *out, void *stream);
+
+/* Adds loss_scale * d(mean squared error)/d(parameter) to gradients and the
+ * mean squared error to loss[0]. */
+int rexlib_hip_mlp_backward(const rexlib_hip_array *features, const rexlib_hip_array *target,
+	const rexlib_hip_array *parameters_f16, const rexlib_hip_array *parameters_f32,
+	const rexlib_hip_array *gradients, const rexlib_hip_array *loss, double loss_scale, void *stream);
+
+/* One Adam step (step counts from 1) on the float32 parameters from
+ * gradients / loss_scale; refreshes the float16 copy and clears the gradients. */
+int rexlib_hip_mlp_adam(const rexlib_hip_array *parameters_f32, const rexlib_hip_array *parameters_f16,
+	const rexlib_hip_array *gradients, const rexlib_hip_array *first_moment,
+	const rexlib_hip_array *second_moment, double learning_rate, double beta1, double beta2, double epsilon,
+	int32_t step, double loss_scale, void *stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
+
 
 #endif /* REXLIB_HIP_H */

@@ -366,3 +366,71 @@ def rel_l2(a, b):
     a = np.asarray(a, dtype=np.complex128).ravel()
     b = np.asarray(b, dtype=np.complex128).ravel()
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# ---------------------------------------------------------------------------
+# Density compensation and its features (no reference function; restated on
+# the full periodic spectrum, which the half spectrum stands for)
+# ---------------------------------------------------------------------------
+
+def full_from_half(half, extent):
+    """The full (N, N, N) real field a half spectrum (N, N, N/2+1) of a real
+    field stands for: element (z, y, x) with x beyond the half is (-z, -y, N-x)."""
+    out = np.empty((extent, extent, extent), dtype=half.dtype)
+    h = extent // 2 + 1
+    out[:, :, :h] = half
+    for x in range(h, extent):
+        out[:, :, x] = np.roll(np.roll(half[::-1, ::-1, extent - x], 1, axis=0), 1, axis=1)
+    return out
+
+
+def density_compensation(den, passes=10, floor=1e-6):
+    """From W = 1, W <- W / max(K3 * (W den), floor) with K3 = (1/6, 2/3, 1/6)
+    per axis, periodic, in double; W = 0 where den <= 0.
+
+    The fields live on the stored half grid: every pass expands W and den to
+    the full grid through the Hermitian mirror of the stored half, so that a
+    neighbour beyond the half always reads the mirrored stored value."""
+    extent = den.shape[0]
+    h = extent // 2 + 1
+    d_half = np.asarray(den, dtype=np.float64)
+    d = full_from_half(d_half, extent)
+    w_half = np.ones_like(d_half)
+    t = {-1: 1 / 6, 0: 2 / 3, 1: 1 / 6}
+    for _ in range(passes):
+        prod = full_from_half(w_half, extent) * d
+        c = np.zeros_like(prod)
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    c += t[dz] * t[dy] * t[dx] * np.roll(prod, (-dz, -dy, -dx), axis=(0, 1, 2))
+        w_half = np.where(d_half > 0, w_half / np.maximum(c[:, :, :h], floor), 0.0)
+    return w_half
+
+
+def compensation_features(den):
+    """(N, N, N/2+1, 10): |g|/(N/2), log1p(den), and log1p of the mean of den
+    over the 7 neighbours of each octant, octants ordered (z, y, x) from
+    (-,-,-) with x fastest."""
+    extent = den.shape[0]
+    h = extent // 2 + 1
+    d = np.maximum(full_from_half(np.asarray(den, dtype=np.float64), extent), 0.0)
+    f = np.fft.fftfreq(extent) * extent
+    fx = np.arange(h)
+    r = np.sqrt(f[:, None, None] ** 2 + f[None, :, None] ** 2 + fx[None, None, :] ** 2) / (extent / 2)
+    out = np.zeros((extent, extent, h, 10))
+    out[..., 0] = r
+    out[..., 1] = np.log1p(d[:, :, :h])
+    b = 0
+    for oz in (-1, 1):
+        for oy in (-1, 1):
+            for ox in (-1, 1):
+                s = np.zeros_like(d)
+                for c in range(1, 8):
+                    dx = ox if c & 1 else 0
+                    dy = oy if c & 2 else 0
+                    dz = oz if c & 4 else 0
+                    s += np.roll(d, (-dz, -dy, -dx), axis=(0, 1, 2))
+                out[..., 2 + b] = np.log1p(s[:, :, :h] / 7.0)
+                b += 1
+    return out

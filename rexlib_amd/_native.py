@@ -67,6 +67,12 @@ EXPORTS = [
     "rexlib_hip_normalize",
     "rexlib_hip_gridding_correct",
     "rexlib_hip_shell_sums",
+    "rexlib_hip_density_compensation",
+    "rexlib_hip_compensation_features",
+    "rexlib_hip_mlp_parameter_count",
+    "rexlib_hip_mlp_forward",
+    "rexlib_hip_mlp_backward",
+    "rexlib_hip_mlp_adam",
 ]
 
 _lib = None
@@ -105,8 +111,16 @@ def library() -> ctypes.CDLL:
     lib.rexlib_hip_normalize.argtypes = [A, A, ctypes.c_double, A, vp]
     lib.rexlib_hip_gridding_correct.argtypes = [A, W, A, vp]
     lib.rexlib_hip_shell_sums.argtypes = [A, A, vp, vp]
+    d = ctypes.c_double
+    lib.rexlib_hip_density_compensation.argtypes = [A, ctypes.c_int32, d, A, vp]
+    lib.rexlib_hip_compensation_features.argtypes = [A, A, vp]
+    lib.rexlib_hip_mlp_parameter_count.argtypes = []
+    lib.rexlib_hip_mlp_forward.argtypes = [A, A, A, A, vp]
+    lib.rexlib_hip_mlp_backward.argtypes = [A, A, A, A, A, A, d, vp]
+    lib.rexlib_hip_mlp_adam.argtypes = [A, A, A, A, A, d, d, d, d, ctypes.c_int32, d, vp]
     for name in EXPORTS[5:]:
         getattr(lib, name).restype = ctypes.c_int
+    lib.rexlib_hip_mlp_parameter_count.restype = ctypes.c_int32
     _lib = lib
     return lib
 

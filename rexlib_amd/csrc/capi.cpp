@@ -409,6 +409,41 @@ void expect_dtype(const rexlib_hip_array &a, int dtype, const char *what)
 	}
 }
 
+bool contiguous(const rexlib_hip_array &a)
+{
+	int64_t expected = 1;
+	for (int i = a.rank - 1; i >= 0; --i)
+	{
+		if (a.extents[i] != 1 && a.strides[i] != expected)
+		{
+			return false;
+		}
+		expected *= a.extents[i];
+	}
+	return true;
+}
+
+int64_t check_mlp_operands(const rexlib_hip_array *features, const rexlib_hip_array *params_h,
+	const rexlib_hip_array *params_f)
+{
+	check_array(features, "mlp features", 2);
+	check_array(params_h, "mlp float16 parameters", 1);
+	check_array(params_f, "mlp float32 parameters", 1);
+	expect_dtype(*features, REXLIB_HIP_FLOAT16, "mlp features");
+	expect_dtype(*params_h, REXLIB_HIP_FLOAT16, "mlp float16 parameters");
+	expect_dtype(*params_f, REXLIB_HIP_FLOAT32, "mlp float32 parameters");
+	const int64_t p = rexlib_hip::mlp_parameter_count();
+	if (features->rank != 2 || features->extents[1] != 16 || !contiguous(*features))
+	{
+		fail(REXLIB_HIP_INVALID_ARGUMENT, "mlp: features are a contiguous (rows, 16) float16 array");
+	}
+	if (params_h->extents[0] != p || params_f->extents[0] != p || !contiguous(*params_h) || !contiguous(*params_f))
+	{
+		fail(REXLIB_HIP_INVALID_ARGUMENT, "mlp: parameters hold rexlib_hip_mlp_parameter_count() elements");
+	}
+	return features->extents[0];
+}
+
 } // anonymous namespace
 
 } // namespace rexlib_hip
@@ -765,6 +800,137 @@ int rexlib_hip_insert_ctf_images(const rexlib_hip_array *numerator, const rexlib
 			{numerator, denominator, &slices, transforms, radial_weights, ctf, numerator, denominator},
 			window, optics, pixel_size, insertion_schedule_from_env(), s};
 		insert(r);
+	});
+}
+
+int rexlib_hip_density_compensation(const rexlib_hip_array *denominator, int32_t passes, double floor_value,
+	const rexlib_hip_array *weights, void *stream)
+{
+	return guarded([&]
+	{
+		check_array(denominator, "density_compensation denominator", 3);
+		check_array(weights, "density_compensation weights", 3);
+		expect_dtype(*denominator, REXLIB_HIP_FLOAT32, "density_compensation denominator");
+		expect_dtype(*weights, REXLIB_HIP_FLOAT32, "density_compensation weights");
+		check_same_shape(*denominator, *weights, "density_compensation weights");
+		const int64_t big = core_extent(*denominator, 3);
+		expect_core(*denominator, "density_compensation denominator", {big, big, big / 2 + 1});
+		if (denominator->rank != 3 || !contiguous(*denominator) || !contiguous(*weights))
+		{
+			fail(REXLIB_HIP_UNSUPPORTED, "density_compensation: one contiguous (N, N, N/2+1) grid");
+		}
+		if (passes < 0)
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "density_compensation: the number of passes must be 0 or more");
+		}
+		if (!(floor_value > 0.0) || !std::isfinite(floor_value))
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "density_compensation: the floor must be finite and positive");
+		}
+		const hipStream_t s = as_stream(stream);
+		const int64_t count = big * big * (big / 2 + 1);
+		float *scratch = static_cast<float *>(workspace_area_bytes(area_compensation,
+			sizeof(float) * static_cast<std::size_t>(count), s));
+		launch_compensation(static_cast<const float *>(denominator->data), static_cast<float *>(weights->data),
+			scratch, static_cast<int>(big), passes, static_cast<float>(floor_value), s);
+	});
+}
+
+int rexlib_hip_compensation_features(const rexlib_hip_array *denominator, const rexlib_hip_array *features,
+	void *stream)
+{
+	return guarded([&]
+	{
+		check_array(denominator, "compensation_features denominator", 3);
+		check_array(features, "compensation_features output", 4);
+		expect_dtype(*denominator, REXLIB_HIP_FLOAT32, "compensation_features denominator");
+		expect_dtype(*features, REXLIB_HIP_FLOAT16, "compensation_features output");
+		const int64_t big = core_extent(*denominator, 3);
+		expect_core(*denominator, "compensation_features denominator", {big, big, big / 2 + 1});
+		expect_core(*features, "compensation_features output", {big, big, big / 2 + 1, 16});
+		if (denominator->rank != 3 || !contiguous(*denominator) || !contiguous(*features))
+		{
+			fail(REXLIB_HIP_UNSUPPORTED, "compensation_features: contiguous operands of one grid");
+		}
+		launch_features(static_cast<const float *>(denominator->data), features->data, static_cast<int>(big),
+			as_stream(stream));
+	});
+}
+
+int32_t rexlib_hip_mlp_parameter_count(void)
+{
+	return rexlib_hip::mlp_parameter_count();
+}
+
+int rexlib_hip_mlp_forward(const rexlib_hip_array *features, const rexlib_hip_array *parameters_f16,
+	const rexlib_hip_array *parameters_f32, const rexlib_hip_array *out, void *stream)
+{
+	return guarded([&]
+	{
+		const int64_t rows = check_mlp_operands(features, parameters_f16, parameters_f32);
+		check_array(out, "mlp output", 1);
+		expect_dtype(*out, REXLIB_HIP_FLOAT32, "mlp output");
+		if (out->rank != 1 || out->extents[0] != rows || !contiguous(*out))
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "mlp_forward: the output holds one float32 per feature row");
+		}
+		launch_mlp_forward(features->data, parameters_f16->data, static_cast<const float *>(parameters_f32->data),
+			static_cast<float *>(out->data), rows, as_stream(stream));
+	});
+}
+
+int rexlib_hip_mlp_backward(const rexlib_hip_array *features, const rexlib_hip_array *target,
+	const rexlib_hip_array *parameters_f16, const rexlib_hip_array *parameters_f32, const rexlib_hip_array *gradients,
+	const rexlib_hip_array *loss, double loss_scale, void *stream)
+{
+	return guarded([&]
+	{
+		const int64_t rows = check_mlp_operands(features, parameters_f16, parameters_f32);
+		check_array(target, "mlp target", 1);
+		check_array(gradients, "mlp gradients", 1);
+		check_array(loss, "mlp loss", 1);
+		expect_dtype(*target, REXLIB_HIP_FLOAT32, "mlp target");
+		expect_dtype(*gradients, REXLIB_HIP_FLOAT32, "mlp gradients");
+		expect_dtype(*loss, REXLIB_HIP_FLOAT32, "mlp loss");
+		if (target->extents[0] != rows || gradients->extents[0] != rexlib_hip::mlp_parameter_count() ||
+			loss->extents[0] < 1 || !contiguous(*target) || !contiguous(*gradients))
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "mlp_backward: target per row, gradients per parameter, one loss");
+		}
+		launch_mlp_backward(features->data, static_cast<const float *>(target->data), parameters_f16->data,
+			static_cast<const float *>(parameters_f32->data), static_cast<float *>(gradients->data),
+			static_cast<float *>(loss->data), rows, static_cast<float>(loss_scale), as_stream(stream));
+	});
+}
+
+int rexlib_hip_mlp_adam(const rexlib_hip_array *parameters_f32, const rexlib_hip_array *parameters_f16,
+	const rexlib_hip_array *gradients, const rexlib_hip_array *first_moment, const rexlib_hip_array *second_moment,
+	double learning_rate, double beta1, double beta2, double epsilon, int32_t step, double loss_scale, void *stream)
+{
+	return guarded([&]
+	{
+		const int64_t p = rexlib_hip::mlp_parameter_count();
+		const rexlib_hip_array *f32[] = {parameters_f32, gradients, first_moment, second_moment};
+		for (const auto *a : f32)
+		{
+			check_array(a, "mlp_adam operand", 1);
+			expect_dtype(*a, REXLIB_HIP_FLOAT32, "mlp_adam operand");
+			if (a->extents[0] != p || !contiguous(*a))
+			{
+				fail(REXLIB_HIP_INVALID_ARGUMENT, "mlp_adam: one contiguous float32 per parameter");
+			}
+		}
+		check_array(parameters_f16, "mlp_adam float16 parameters", 1);
+		expect_dtype(*parameters_f16, REXLIB_HIP_FLOAT16, "mlp_adam float16 parameters");
+		if (step < 1)
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "mlp_adam: steps count from 1");
+		}
+		launch_adam(static_cast<float *>(parameters_f32->data), parameters_f16->data,
+			static_cast<float *>(gradients->data), static_cast<float *>(first_moment->data),
+			static_cast<float *>(second_moment->data), static_cast<float>(learning_rate), static_cast<float>(beta1),
+			static_cast<float>(beta2), static_cast<float>(epsilon), step, static_cast<float>(loss_scale),
+			as_stream(stream));
 	});
 }
 

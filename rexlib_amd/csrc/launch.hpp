@@ -14,7 +14,8 @@ enum workspace_area : int
 	area_spectra = 1,
 	area_fft = 2,
 	area_prepared = 3,
-	area_count = 4
+	area_compensation = 4,
+	area_count = 5
 };
 
 void *workspace_area_bytes(int area, std::size_t bytes, hipStream_t stream);
@@ -47,6 +48,17 @@ void launch_normalize(const float2 *num, const float *den, float2 *out, int64_t 
 
 void launch_gridding(const float *in, float *out, int big, int64_t volumes, int profile, double half_width,
 	double shape, hipStream_t stream);
+
+int mlp_parameter_count();
+void launch_compensation(const float *den, float *w, float *scratch, int big, int passes, float floor_value,
+	hipStream_t stream);
+void launch_features(const float *den, void *features, int big, hipStream_t stream);
+void launch_mlp_forward(const void *features, const void *params_h, const float *params_f, float *out, int64_t count,
+	hipStream_t stream);
+void launch_mlp_backward(const void *features, const float *target, const void *params_h, const float *params_f,
+	float *grads, float *loss, int64_t count, float loss_scale, hipStream_t stream);
+void launch_adam(float *params, void *params_h, float *grads, float *m, float *v, float lr, float beta1, float beta2,
+	float eps, int step, float loss_scale, hipStream_t stream);
 
 void launch_shell_sums(const float2 *a, const float2 *b, double *sums, int big, hipStream_t stream);
 

@@ -1,0 +1,92 @@
+"""The double-precision oracle of the compensation perceptron and of the
+density compensation, pinned on the CPU."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp
+
+P = sum(int(np.prod(s)) for _, s in mlp.SHAPES)
+
+
+def random_parameters(seed):
+    rng = np.random.default_rng(seed)
+    p = {name: rng.normal(0, 0.3, shape) for name, shape in mlp.SHAPES}
+    p["w1"][10:] = 0.0
+    p["w4"][:, 1:] = 0.0
+    p["b4"][1:] = 0.0
+    return mlp.pack(p)
+
+
+def test_gradients_match_finite_differences():
+    rng = np.random.default_rng(1)
+    flat = random_parameters(2)
+    x = np.zeros((9, 16))
+    x[:, :10] = rng.normal(size=(9, 10))
+    y = rng.normal(size=9)
+    _, g = mlp.gradients(flat, x, y)
+    eps = 1e-6
+    live = np.flatnonzero(np.abs(flat) > 0)
+    for i in rng.choice(live, 60, replace=False):
+        a = flat.copy()
+        b = flat.copy()
+        a[i] += eps
+        b[i] -= eps
+        fd = (mlp.loss(a, x, y) - mlp.loss(b, x, y)) / (2 * eps)
+        assert g[i] == pytest.approx(fd, rel=1e-5, abs=1e-8)
+
+
+def test_one_unit_network_closed_form():
+    # A single live path: out = w4 * relu(w3 * relu(w2 * relu(w1 x0 + b1) ...)).
+    p = {name: np.zeros(shape) for name, shape in mlp.SHAPES}
+    p["w1"][0, 0], p["w2"][0, 0], p["w3"][0, 0], p["w4"][0, 0] = 2.0, 3.0, 0.5, 4.0
+    p["b4"][0] = 1.0
+    flat = mlp.pack(p)
+    x = np.zeros((1, 16))
+    x[0, 0] = 1.5
+    out, _ = mlp.forward(flat, x)
+    assert out[0] == pytest.approx(4.0 * 0.5 * 3.0 * 2.0 * 1.5 + 1.0)
+    _, g = mlp.gradients(flat, x, np.array([0.0]))
+    gp = mlp.unpack(g)
+    d = 2 * out[0]
+    assert gp["b4"][0] == pytest.approx(d)
+    assert gp["w1"][0, 0] == pytest.approx(d * 4.0 * 0.5 * 3.0 * 1.5)
+
+
+def test_density_compensation_flattens_uniform_sampling():
+    # A constant density converges to W = 1 / den after one pass.
+    den = np.full((8, 8, 5), 4.0)
+    w = oracle.density_compensation(den, passes=3)
+    np.testing.assert_allclose(w, 0.25)
+
+
+def test_density_compensation_fixed_point():
+    rng = np.random.default_rng(0)
+    den = rng.uniform(0.5, 2.0, (8, 8, 5))
+    w = oracle.density_compensation(den, passes=60)
+    full_w = oracle.full_from_half(w, 8)
+    full_d = oracle.full_from_half(den, 8)
+    prod = full_w * full_d
+    t = {-1: 1 / 6, 0: 2 / 3, 1: 1 / 6}
+    c = sum(t[a] * t[b] * t[cc] * np.roll(prod, (-a, -b, -cc), axis=(0, 1, 2))
+            for a in (-1, 0, 1) for b in (-1, 0, 1) for cc in (-1, 0, 1))
+    np.testing.assert_allclose(c, 1.0, atol=1e-3)
+
+
+def test_full_from_half_is_hermitian_for_a_real_field():
+    rng = np.random.default_rng(3)
+    full = rng.normal(size=(6, 6, 6))
+    half_spec = np.fft.rfftn(full)
+    spec_full = np.fft.fftn(full)
+    got = oracle.full_from_half(np.abs(half_spec), 6)
+    np.testing.assert_allclose(got, np.abs(spec_full), atol=1e-9)
+
+
+def test_features_at_the_origin():
+    den = np.zeros((8, 8, 5))
+    den[0, 0, 0] = np.e - 1
+    f = oracle.compensation_features(den)
+    assert f[0, 0, 0, 0] == 0.0
+    assert f[0, 0, 0, 1] == pytest.approx(1.0)
+    # The neighbour (1, 0, 0) sees the origin in the octants with ox = -1.
+    assert f[0, 0, 1, 2] == pytest.approx(np.log1p((np.e - 1) / 7))
