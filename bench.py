@@ -42,6 +42,20 @@ def algorithmic_bytes_per_image(n, big, launch, pixel_bytes=4):
     return n * n * pixel_bytes + 64 + 24.0 * big * big * (big // 2 + 1) / launch
 
 
+def traffic_per_launch(args):
+    """HBM bytes per brick-kernel launch of this workload, from the committed
+    rocprofv3 FETCH_SIZE and WRITE_SIZE passes (KiB per dispatch, scaled by the
+    factor a copy kernel of known size gives), or None when absent."""
+    path = os.path.join(ROOT, "profiles", "brick_kernel_256_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        record = json.load(f)
+    if record.get("n") != args.n or record.get("N") != args.big or record.get("launch") != args.launch:
+        return None
+    return record["bytes_per_launch"]
+
+
 def make_workload(args, torch):
     from rexlib_amd import synthetic
     t0 = time.time()
@@ -143,6 +157,9 @@ def main():
     parser.add_argument("--cpu-seconds", type=float, default=12.0)
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-parity", action="store_true")
+    parser.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                        help="before the workload, copy a float32 buffer of MIB MiB three times with an elementwise "
+                             "kernel (known traffic, to scale hardware byte counters)")
     args = parser.parse_args()
 
     if args.gpus != 1:
@@ -153,9 +170,17 @@ def main():
     import torch
 
     import rexlib_amd
-    from rexlib_amd import em
+    from rexlib_amd import _native, em
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration * (1 << 18), device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads and writes MIB MiB
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 copies of {args.copy_calibration} MiB")
+        del src, dst
     stack = make_workload(args, torch)
     n, big, L = args.n, args.big, args.launch
     h = n // 2 + 1
@@ -169,52 +194,54 @@ def main():
     optics = em.CtfOptics(300.0, 2.7, 0.1)
     launches = [(s, min(s + L, args.images)) for s in range(0, args.images, L)]
 
-    insert_events = []
-
-    def step(record):
+    def step():
         num.zero_()
         den.zero_()
         for s, e in launches:
             out = spectra[: e - s]
             rexlib_amd.rfft2(stack.images[s:e], out=out)
-            if record:
-                a = torch.cuda.Event(enable_timing=True)
-                b = torch.cuda.Event(enable_timing=True)
-                a.record()
             em.insert_ctf_slices(num, den, out, transforms[s:e], ctf[s:e], radial, window, optics, 1.5,
                                  numerator_out=num, denominator_out=den)
-            if record:
-                b.record()
-                insert_events.append((a, b, e - s))
         volume = rexlib_amd.irfft3(em.normalize(num, den, 1e-3))
         return em.gridding_correct(volume, window, out=volume)
 
     for _ in range(args.warmup):
-        step(False)
+        step()
     torch.cuda.synchronize()
+    # The library brackets each launch of its insertion kernels with HIP
+    # events on the stream they run on (torch's current stream here).
+    _native.set_kernel_timing(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(True)
+        step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    brick_ms, brick_launches = _native.kernel_timing(_native.KERNEL_BRICK)
+    prepare_ms, prepare_launches = _native.kernel_timing(_native.KERNEL_PREPARE)
+    _native.set_kernel_timing(False)
     ms_per_step = 1e3 * elapsed / args.steps
     value = args.images * args.steps / elapsed
     log(f"{args.steps} steps of {args.images} images: {elapsed:.3f} s, {value:,.0f} images/s")
 
-    full = [(a.elapsed_time(b), c) for a, b, c in insert_events if c == L]
-    avg_ms = float(np.mean([t for t, _ in full])) if full else float("nan")
+    # Average over launches; the final partial launch holds images % L images.
+    images_timed = args.images * args.steps
     b_img = algorithmic_bytes_per_image(n, big, L)
-    achieved = b_img * L / (avg_ms * 1e-3) / 1e9 if full else None
+    avg_ms = brick_ms / brick_launches if brick_launches else float("nan")
+    images_per_launch = images_timed / brick_launches if brick_launches else float("nan")
+    achieved = b_img * images_timed / (brick_ms * 1e-3) / 1e9 if brick_launches else None
     roofline = {
         "bound": "hbm",
-        "kernel": "insertion (settle + scatter kernels) per launch of 4096 images",
+        "kernel": "brick_kernel (LDS-owned brick insertion, CTF + trilinear)",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS if achieved else None,
-        "traffic": None,
+        "traffic": traffic_per_launch(args),
         "launch_ms": avg_ms,
+        "launches": brick_launches,
+        "images_per_launch": images_per_launch,
         "bytes_per_image": b_img,
+        "prepare_kernel_ms": prepare_ms / prepare_launches if prepare_launches else None,
         "pipeline_frac": value * b_img / (HBM_PEAK_GBS * 1e9),
     }
 

@@ -102,6 +102,24 @@ int32_t rexlib_hip_device_count(void);
 void rexlib_hip_set_insertion_tuning(int32_t extent_y, int32_t extent_z, int32_t share);
 void rexlib_hip_get_insertion_tuning(int32_t *extent_y, int32_t *extent_z, int32_t *share);
 
+/*
+ * Event timing of single kernels, for measurement. When switched on, the
+ * library brackets each launch of the named kernels with two HIP events on
+ * the launch stream (not while the stream is being captured); switching on or
+ * off discards earlier records. rexlib_hip_kernel_timing waits for the
+ * recorded launches of `kernel` and returns their total time and count.
+ * No reference counterpart (the reference times whole calls from outside).
+ */
+typedef enum rexlib_hip_kernel
+{
+    REXLIB_HIP_KERNEL_BRICK = 0,   /* brick insertion kernel */
+    REXLIB_HIP_KERNEL_PREPARE = 1, /* per-sample preparation for the brick kernel */
+    REXLIB_HIP_KERNEL_SCATTER = 2  /* atomic scatter insertion kernel */
+} rexlib_hip_kernel;
+
+int rexlib_hip_set_kernel_timing(int32_t enable);
+int rexlib_hip_kernel_timing(int32_t kernel, double *total_ms, int64_t *launches);
+
 /* ------------------------------------------------------------------------- */
 /* Slice insertion — the hot path                                            */
 /* ------------------------------------------------------------------------- */

@@ -73,6 +73,8 @@ EXPORTS = [
     "rexlib_hip_mlp_forward",
     "rexlib_hip_mlp_backward",
     "rexlib_hip_mlp_adam",
+    "rexlib_hip_set_kernel_timing",
+    "rexlib_hip_kernel_timing",
 ]
 
 _lib = None
@@ -118,6 +120,8 @@ def library() -> ctypes.CDLL:
     lib.rexlib_hip_mlp_forward.argtypes = [A, A, A, A, vp]
     lib.rexlib_hip_mlp_backward.argtypes = [A, A, A, A, A, A, d, vp]
     lib.rexlib_hip_mlp_adam.argtypes = [A, A, A, A, A, d, d, d, d, ctypes.c_int32, d, vp]
+    lib.rexlib_hip_set_kernel_timing.argtypes = [ctypes.c_int32]
+    lib.rexlib_hip_kernel_timing.argtypes = [ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_int64)]
     for name in EXPORTS[5:]:
         getattr(lib, name).restype = ctypes.c_int
     lib.rexlib_hip_mlp_parameter_count.restype = ctypes.c_int32
@@ -156,3 +160,19 @@ def check(status: int) -> None:
     if status == UNSUPPORTED:
         raise UnsupportedOperationError(message)
     raise RuntimeError(message)
+
+
+KERNEL_BRICK, KERNEL_PREPARE, KERNEL_SCATTER = 0, 1, 2
+
+
+def set_kernel_timing(enable: bool) -> None:
+    """Bracket each launch of the insertion kernels with HIP events on its stream."""
+    check(library().rexlib_hip_set_kernel_timing(1 if enable else 0))
+
+
+def kernel_timing(kernel: int) -> tuple[float, int]:
+    """(total milliseconds, launches) recorded for `kernel` since timing was switched on."""
+    total = ctypes.c_double()
+    launches = ctypes.c_int64()
+    check(library().rexlib_hip_kernel_timing(int(kernel), ctypes.byref(total), ctypes.byref(launches)))
+    return total.value, launches.value

@@ -496,6 +496,27 @@ void rexlib_hip_get_insertion_tuning(int32_t *extent_y, int32_t *extent_z, int32
 	}
 }
 
+int rexlib_hip_set_kernel_timing(int32_t enable)
+{
+	return guarded([&] { rexlib_hip::set_kernel_timing(enable != 0); });
+}
+
+int rexlib_hip_kernel_timing(int32_t kernel, double *total_ms, int64_t *launches)
+{
+	return guarded([&]
+		{
+			if (!total_ms || !launches)
+			{
+				throw rexlib_hip::error(REXLIB_HIP_INVALID_ARGUMENT, "kernel_timing: null output");
+			}
+			if (kernel < REXLIB_HIP_KERNEL_BRICK || kernel > REXLIB_HIP_KERNEL_SCATTER)
+			{
+				throw rexlib_hip::error(REXLIB_HIP_INVALID_ARGUMENT, "kernel_timing: unknown kernel id");
+			}
+			rexlib_hip::kernel_timing(kernel, total_ms, launches);
+		});
+}
+
 int32_t rexlib_hip_device_count(void)
 {
 	int count = 0;

@@ -1282,12 +1282,17 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 		for (int64_t begin = 0; begin < p.map.reduced_count; begin += chunk_count)
 		{
 			const slice_chunk chunk{begin, std::min(chunk_count, p.map.reduced_count - begin)};
+			const int prepare_token = timing_begin(REXLIB_HIP_KERNEL_PREPARE, stream);
 			dispatch_prepare(p, chunk, prepared, stream);
+			timing_end(prepare_token, stream);
+			const int brick_token = timing_begin(REXLIB_HIP_KERNEL_BRICK, stream);
 			dispatch_brick(p, chunk, prepared, tasks, count, reach, stream);
+			timing_end(brick_token, stream);
 		}
 	}
 	else
 	{
+		const int scatter_token = timing_begin(REXLIB_HIP_KERNEL_SCATTER, stream);
 		switch (p.variant)
 		{
 		case variant_plain:
@@ -1300,6 +1305,7 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 			dispatch_scatter<variant_ctf>(p, stream);
 			break;
 		}
+		timing_end(scatter_token, stream);
 	}
 	REXLIB_HIP_CHECK(hipGetLastError());
 }

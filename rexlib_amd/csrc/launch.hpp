@@ -32,6 +32,12 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 	const float *den_in, const int64_t *den_in_strides, bool in_place, double shape, hipStream_t stream);
 
 void set_brick_tuning(int extent_y, int extent_z, int share);
+
+// Event timing of single kernels (ids as REXLIB_HIP_KERNEL_*).
+void set_kernel_timing(bool enable);
+int timing_begin(int kernel, hipStream_t stream);
+void timing_end(int token, hipStream_t stream);
+void kernel_timing(int kernel, double *total_ms, int64_t *launches);
 void get_brick_tuning(int *extent_y, int *extent_z, int *share);
 
 void launch_rfft_rows(int kind, const void *in, const line_set &in_rows, int m, const float *scales,
