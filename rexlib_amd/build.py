@@ -1,8 +1,15 @@
-"""Build the gfx950 backend library in-tree: rexlib_amd/librexlib_hip.so.
+"""Build the gfx950 backend in-tree.
 
-One hipcc invocation compiles every HIP and C++ source of rexlib_amd/csrc for
-gfx950 and links the shared object. Only the C ABI (rexlib_hip_*) and the
-plugin entry points (rexlib_get_plugin*) are exported.
+- rexlib_amd/librexlib_hip.so: one hipcc invocation compiles every HIP and C++
+  source of rexlib_amd/csrc for gfx950; only the C ABI (rexlib_hip_*) is
+  exported.
+- rexlib_amd/librexlib_host.so: the C++ core of the mirrored rexlib API
+  (rexlib_amd/host/core.cpp): services, plugin loader, arrays, the eager
+  dispatcher and the em / Fourier functions.
+- rexlib_amd/librexlib-hip.so: the plugin (rexlib_amd/host/hip_plugin.cpp),
+  exporting rexlib_get_plugin_compatibility and rexlib_get_plugin; it runs
+  librexlib_hip.so's kernels through the C ABI.
+- tests/plugin_driver: the C++ caller the plugin tests run.
 """
 from __future__ import annotations
 
@@ -41,6 +48,43 @@ def sources():
     return [os.path.join(CSRC, f) for f in names if f.endswith((".hip", ".cpp"))]
 
 
+ROOT = os.path.dirname(HERE)
+HOST = os.path.join(HERE, "host")
+HOST_LIBRARY = os.path.join(HERE, "librexlib_host.so")
+PLUGIN = os.path.join(HERE, "librexlib-hip.so")
+DRIVER = os.path.join(ROOT, "tests", "plugin_driver")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+CXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+
+
+def _run(cmd, what, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout + proc.stderr)
+        raise RuntimeError(f"failed to build {what}")
+
+
+def build_host(verbose: bool = False) -> list:
+    """The mirrored rexlib core, the plugin and the test driver (host code, g++)."""
+    cxx = os.environ.get("CXX", "g++")
+    _run([cxx, *CXX_FLAGS, "-fPIC", "-shared", "-fvisibility=hidden", "-I", HOST,
+          os.path.join(HOST, "core.cpp"), "-ldl", "-o", HOST_LIBRARY + ".tmp"], "librexlib_host.so", verbose)
+    os.replace(HOST_LIBRARY + ".tmp", HOST_LIBRARY)
+    _run([cxx, *CXX_FLAGS, "-fPIC", "-shared", "-fvisibility=hidden", "-D__HIP_PLATFORM_AMD__",
+          "-I", HOST, "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROCM, "include"),
+          os.path.join(HOST, "hip_plugin.cpp"), "-L", HERE, "-lrexlib_host", "-l:librexlib_hip.so",
+          "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM}/lib",
+          "-Wl,--no-undefined", "-o", PLUGIN + ".tmp"], "librexlib-hip.so", verbose)
+    os.replace(PLUGIN + ".tmp", PLUGIN)
+    _run([cxx, *CXX_FLAGS, "-I", HOST, os.path.join(ROOT, "tests", "plugin_driver.cpp"), "-L", HERE,
+          "-lrexlib_host", "-ldl", "-Wl,-rpath,$ORIGIN/../rexlib_amd", "-o", DRIVER + ".tmp"], "plugin_driver",
+         verbose)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return [HOST_LIBRARY, PLUGIN, DRIVER]
+
+
 def build(verbose: bool = False) -> str:
     exports = os.path.join(CSRC, "exports.map")
     tmp = LIBRARY + ".tmp"
@@ -53,6 +97,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(proc.stdout + proc.stderr)
         raise RuntimeError("hipcc failed to build librexlib_hip.so")
     os.replace(tmp, LIBRARY)
+    build_host(verbose)
     return LIBRARY
 
 

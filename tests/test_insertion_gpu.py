@@ -208,3 +208,23 @@ def test_production_like_sizes_match_the_reference_kernels(variant, schedule):
             ref = oracle.insert_f64(variant, inputs, w, pixel_size=PIXEL_SIZE)
         assert oracle.rel_l2(num, ref[0]) <= 1e-5
         assert oracle.rel_l2(den, ref[1]) <= 1e-5
+
+
+def test_kernel_timing_records_each_launch(schedule):
+    """Event timing brackets every launch of the schedule's insertion kernel
+    and of nothing else; switching it off stops the records."""
+    from rexlib_amd import _native as N
+    inputs = make_case("ctf", 32, 32, 9, slices=40)
+    kernel = N.KERNEL_BRICK if schedule == "brick" else N.KERNEL_SCATTER
+    other = N.KERNEL_SCATTER if schedule == "brick" else N.KERNEL_BRICK
+    N.set_kernel_timing(True)
+    try:
+        for _ in range(3):
+            hip_insert("ctf", inputs, oracle.Window.linear(), PIXEL_SIZE)
+        total, launches = N.kernel_timing(kernel)
+        assert launches == 3 and total > 0
+        assert N.kernel_timing(other) == (0.0, 0)
+    finally:
+        N.set_kernel_timing(False)
+    hip_insert("ctf", inputs, oracle.Window.linear(), PIXEL_SIZE)
+    assert N.kernel_timing(kernel) == (0.0, 0)

@@ -1,0 +1,79 @@
+"""The plugin boundary on the CPU: librexlib-hip.so exports exactly the entry
+points include/rexlib_hip_plugin.h declares, states the compatibility record
+of its build, registers the "hip" backend (zero devices without a GPU, which
+is not an error) and its program builders, and follows the reference loader's
+compatibility rules (src/core/plugin_compatibility_check.cpp) and error
+conventions (include/rexlib/core/dispatch/program_builder.hpp:47-99:
+builders decline, the service raises unsupported_operation_error)."""
+import ctypes
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PLUGIN = os.path.join(ROOT, "rexlib_amd", "librexlib-hip.so")
+DRIVER = os.path.join(ROOT, "tests", "plugin_driver")
+HEADER = os.path.join(ROOT, "include", "rexlib_hip_plugin.h")
+
+
+def built():
+    if not (os.path.exists(PLUGIN) and os.path.exists(DRIVER)):
+        pytest.fail("the plugin is not built: run `python rexlib_amd/build.py`")
+
+
+def driver(*args, timeout=120):
+    built()
+    proc = subprocess.run([DRIVER, *args], capture_output=True, text=True, timeout=timeout)
+    lines = [json.loads(line) for line in proc.stdout.splitlines() if line.strip()]
+    return proc.returncode, lines, proc.stderr
+
+
+class Record(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint32) for name in (
+        "size", "major", "minor", "patch", "abi_family", "standard_library", "standard_library_abi", "runtime")]
+
+
+def test_exports_are_the_declared_entry_points():
+    built()
+    declared = sorted(set(re.findall(r"\b(rexlib_get_plugin[a-z_]*)\s*\(", open(HEADER).read())))
+    assert declared == ["rexlib_get_plugin", "rexlib_get_plugin_compatibility"]
+    out = subprocess.run(["nm", "-D", "--defined-only", PLUGIN], capture_output=True, text=True, check=True).stdout
+    functions = sorted(line.split()[-1] for line in out.splitlines() if " T " in line)
+    assert functions == declared
+
+
+def test_entry_points_through_ctypes():
+    built()
+    lib = ctypes.CDLL(PLUGIN)
+    lib.rexlib_get_plugin_compatibility.restype = ctypes.POINTER(Record)
+    lib.rexlib_get_plugin.restype = ctypes.c_void_p
+    r = lib.rexlib_get_plugin_compatibility().contents
+    # 32-byte record, rexlib 0.1.0, Itanium ABI, libstdc++ with the C++11 ABI, no MSVC runtime.
+    assert (r.size, r.major, r.minor, r.patch) == (32, 0, 1, 0)
+    assert (r.abi_family, r.standard_library, r.standard_library_abi, r.runtime) == (1, 1, 1, 0)
+    first = lib.rexlib_get_plugin()
+    assert first and lib.rexlib_get_plugin() == first  # one static instance
+
+
+def test_registration_and_rules():
+    rc, lines, err = driver("info", PLUGIN)
+    assert rc == 0, err
+    record, checks, plugin, errors = lines
+    assert record["compatibility"] == [32, 0, 1, 0, 1, 1, 1, 0]
+    # Pre-1.0: same minor required, patch free; short records and other ABIs refused.
+    assert checks == {"check_same": "ok", "check_patch": "ok", "check_minor": "plugin_load_error",
+                      "check_size": "plugin_load_error", "check_abi": "plugin_load_error"}
+    assert plugin["name"] == "rexlib-hip" and plugin["version"] == "0.1.0"
+    assert plugin["backends"] == ["hip"]
+    assert plugin["devices"] >= 0
+    assert plugin["operations"] == [
+        "rexlib::em::insert_slices_operation", "rexlib::em::insert_weighted_slices_operation",
+        "rexlib::em::insert_ctf_slices_operation", "rexlib::ops::rfft_operation", "rexlib::ops::irfft_operation",
+        "rexlib::ops::fill_operation", "rexlib::ops::copy_operation"]
+    assert errors["foreign_queue"] == "unsupported_operation_error"
+    assert errors["bad_pixel_size"] == errors["bad_optics"] == errors["bad_window"] == "invalid_argument"
+    # lambda(300 kV), tests/unitary/src/em/ops/parameters/test_ctf_optics.cpp:40-53.
+    assert errors["wavelength_300kV"] == pytest.approx(0.019687615309, rel=1e-5)

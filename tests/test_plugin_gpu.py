@@ -1,0 +1,72 @@
+"""The plugin boundary on the GPU: a C++ caller of the mirrored rexlib API
+(tests/plugin_driver.cpp) loads librexlib-hip.so, opens hip:0, and runs
+rfft2, insert_ctf_slices in two calls that accumulate in place, and irfft3,
+with no HIP code of its own. Results are checked against numpy's pocketfft
+and the double-precision oracle, within the reference's float32 tolerance
+(rel-L2 1e-4, tests/unitary/src/backends/cpu/builders/em/reconstruction/
+test_slice_insertion_program_builder.cpp:190). Errors cross the boundary as
+the reference's exceptions."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from rexlib_amd import synthetic
+from test_plugin import DRIVER, PLUGIN, driver
+
+pytestmark = pytest.mark.gpu
+
+WINDOWS = {0: oracle.Window.nearest(), 1: oracle.Window.linear(), 2: oracle.Window.kaiser_bessel(1.5, 6.0)}
+
+
+@pytest.mark.parametrize("profile", [1, 0, 2])
+def test_reconstruction_through_the_plugin(tmp_path, profile):
+    rng = np.random.default_rng(40 + profile)
+    count, n, big, split = 24, 32, 48, 10
+    h = n // 2 + 1
+    images = rng.normal(size=(count, n, n)).astype(np.float32)
+    transforms = synthetic.transforms_of(synthetic.uniform_rotations(count, rng)).astype(np.float32)
+    transforms[:, :2, 3] = rng.uniform(-2, 2, (count, 2))
+    ctf = np.stack([rng.uniform(1e4, 3e4, count), rng.uniform(1e4, 3e4, count), rng.uniform(0, np.pi, count),
+                    rng.uniform(0, 0.5, count)], axis=1).astype(np.float32)
+    radial = rng.uniform(0.2, 2.0, h).astype(np.float32)
+    for name, a in [("images", images), ("transforms", transforms), ("ctf", ctf), ("radial", radial)]:
+        a.tofile(tmp_path / f"{name}.bin")
+    w = WINDOWS[profile]
+    meta = dict(count=count, n=n, big=big, split=split, window=profile, half_width=w.half_width, shape=w.shape,
+                voltage=300, cs=2.7, q=0.1, pixel=1.5)
+    (tmp_path / "meta.txt").write_text("".join(f"{k} {v}\n" for k, v in meta.items()))
+    print(f"plugin run: window {profile}, {count} images {n}^2 -> {big}^3", flush=True)
+    rc, lines, err = driver("run", PLUGIN, str(tmp_path))
+    assert rc == 0, (lines, err)
+    assert lines[-1]["run"] == "ok"
+
+    spectra = np.fromfile(tmp_path / "spectra_out.bin", np.complex64).reshape(count, n, h)
+    want = np.fft.rfft2(images.astype(np.float64))
+    assert oracle.rel_l2(spectra, want) < 1e-5
+
+    hb = big // 2 + 1
+    num = np.fromfile(tmp_path / "num_out.bin", np.complex64).reshape(big, big, hb)
+    den = np.fromfile(tmp_path / "den_out.bin", np.float32).reshape(big, big, hb)
+    acc = np.zeros((big, big, hb), np.complex128)
+    ref = oracle.insert_f64("ctf", [acc, acc.real.copy(), want, transforms.astype(np.float64),
+                                    ctf.astype(np.float64), radial.astype(np.float64)], w, pixel_size=1.5)
+    assert oracle.rel_l2(num, ref[0]) <= 1e-4
+    assert oracle.rel_l2(den, ref[1]) <= 1e-4
+
+    volume = np.fromfile(tmp_path / "volume_out.bin", np.float32).reshape(big, big, big)
+    assert oracle.rel_l2(volume, np.fft.irfftn(num.astype(np.complex128), s=(big, big, big), axes=(0, 1, 2))) < 1e-4
+
+
+def test_errors_cross_the_boundary_as_the_reference_exceptions():
+    rc, lines, err = driver("errors", PLUGIN)
+    assert rc == 0, (lines, err)
+    e = lines[-1]
+    assert e["valid"] == "ok"
+    assert e["radial_extent"] == "invalid_argument"
+    assert e["transform_shape"] == "invalid_argument"
+    # float64 and host memory have no HIP program: the service finds no builder.
+    assert e["float64"] == "unsupported_operation_error"
+    assert e["host_operand"] == "unsupported_operation_error"
+    assert e["new_outputs"] == "ok"
