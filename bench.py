@@ -110,6 +110,97 @@ def cpu_baseline(args, stack, threads):
     }
 
 
+def timed(torch, fn, repeats=1):
+    """Wall time of `fn` (ms per call) between device synchronisations, after one untimed call."""
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        fn()
+    torch.cuda.synchronize()
+    return 1e3 * (time.perf_counter() - t0) / repeats
+
+
+def precision_forms(args, stack, transforms, ctf, radial, reference, torch, em, rexlib_amd):
+    """The same reconstruction from images staged in HBM as float32, bfloat16 and
+    per-image scaled float16 (fused rfft2 + insertion per launch), with the
+    minimum FSC to the plain float32 result up to 0.9 Nyquist."""
+    from rexlib_amd import precision
+    n, big, L = args.n, args.big, args.launch
+    window = em.GriddingWindow.linear()
+    optics = em.CtfOptics(300.0, 2.7, 0.1)
+    out = {}
+    for dtype in ("f32", "bf16", "fp16"):
+        staged, scales = precision.stage_images(stack.images, dtype)
+        num = torch.zeros((big, big, big // 2 + 1), dtype=torch.complex64, device="cuda")
+        den = torch.zeros((big, big, big // 2 + 1), dtype=torch.float32, device="cuda")
+
+        def run():
+            num.zero_()
+            den.zero_()
+            for s in range(0, args.images, L):
+                e = min(s + L, args.images)
+                em.insert_ctf_images(num, den, staged[s:e], transforms[s:e], ctf[s:e], radial, window, optics, 1.5,
+                                     image_scales=None if scales is None else scales[s:e], padded=n)
+            volume = rexlib_amd.irfft3(em.normalize(num, den, 1e-3))
+            return em.gridding_correct(volume, window, out=volume)
+
+        ms = timed(torch, run)
+        report = precision.shell_report(num.contiguous(), reference, 0.9)
+        out[dtype] = {"images_per_s": args.images / (ms * 1e-3), "ms": ms,
+                      "min_fsc_to_fp32_to_0.9_nyquist": report["min_fsc_to_cutoff"],
+                      "staged_bytes_per_image": n * n * staged.element_size() + (4 if scales is not None else 0)}
+        del staged, scales, num, den
+    log(f"precision forms: {out}")
+    return out
+
+
+def learned_form(den, torch):
+    """Density compensation of the benchmark's weight volume: the 10-pass
+    iterative estimate against one forward pass of the perceptron (features +
+    MFMA forward over every voxel); untrained weights, since the timing does
+    not depend on their values."""
+    from rexlib_amd import learned
+    model = learned.CompensationModel(seed=0)
+    voxels = den.numel()
+    iterative_ms = timed(torch, lambda: learned.iterative_weights(den), repeats=3)
+    learned_ms = timed(torch, lambda: model.weights(den), repeats=3)
+    rows = learned.features(den).reshape(-1, learned.FEATURES)
+    forward_ms = timed(torch, lambda: model.predict(rows), repeats=3)
+    flops = 2.0 * (10 * 64 + 64 * 64 + 64 * 64 + 64 * 1) * voxels
+    out = {"voxels": voxels, "iterative_ms": iterative_ms, "learned_ms": learned_ms,
+           "mlp_forward_ms": forward_ms, "mlp_forward_tflops": flops / (forward_ms * 1e-3) / 1e12,
+           "mlp_forward_frac_of_fp16_dense_peak": flops / (forward_ms * 1e-3) / 2.5e15}
+    log(f"learned form: {out}")
+    return out
+
+
+def batched_form(torch, count=16, seed=2000):
+    """Reconstructions of mixed box sizes ({64, 96, 128}) and particle counts
+    (log-uniform 500-4000): one HIP graph replay against the same runs one
+    after another."""
+    from rexlib_amd import batched, synthetic
+    rng = np.random.default_rng(seed)
+    recons = []
+    images = 0
+    for i in range(count):
+        n = int(rng.choice([64, 96, 128]))
+        particles = int(np.exp(rng.uniform(np.log(500), np.log(4000))))
+        vol = synthetic.blob_volume(n, 30, n * 0.35, (1.0, 4.0), seed=seed + 10 * i)
+        stack = synthetic.particle_stack(vol, particles, n, seed=seed + 10 * i + 1, snr=float(rng.uniform(0.05, 0.5)),
+                                         device="cuda")
+        recons.append(batched.Reconstruction(stack.images, torch.from_numpy(stack.transforms.astype(np.float32)).cuda(),
+                                             torch.from_numpy(stack.ctf.astype(np.float32)).cuda()))
+        images += particles
+    sequential_ms = timed(torch, lambda: batched.run_sequential(recons))
+    graph = batched.BatchedReconstruction(recons, streams=4).capture()
+    graph_ms = timed(torch, graph.replay, repeats=3)
+    out = {"reconstructions": count, "images": images, "sequential_ms": sequential_ms, "graph_ms": graph_ms,
+           "speedup": sequential_ms / graph_ms, "graph_images_per_s": images / (graph_ms * 1e-3)}
+    log(f"batched form: {out}")
+    return out
+
+
 def parity_sample(args, stack, torch, em, rexlib_amd, count=256):
     """rel-L2 of the GPU accumulators against the reference's float32 kernels on
     the first `count` images."""
@@ -157,6 +248,8 @@ def main():
     parser.add_argument("--cpu-seconds", type=float, default=12.0)
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-parity", action="store_true")
+    parser.add_argument("--no-forms", action="store_true",
+                        help="skip the precision, learned and batched legs reported beside the headline value")
     parser.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                         help="before the workload, copy a float32 buffer of MIB MiB three times with an elementwise "
                              "kernel (known traffic, to scale hardware byte counters)")
@@ -245,6 +338,18 @@ def main():
         "pipeline_frac": value * b_img / (HBM_PEAK_GBS * 1e9),
     }
 
+    forms = None
+    if not args.no_forms:
+        step()
+        torch.cuda.synchronize()
+        reference = num.clone()
+        forms = {
+            "precision": precision_forms(args, stack, transforms, ctf, radial, reference, torch, em, rexlib_amd),
+            "learned": learned_form(den, torch),
+            "batched": batched_form(torch),
+        }
+        del reference
+
     parity = None if args.no_parity else parity_sample(args, stack, torch, em, rexlib_amd)
     if parity:
         log(f"parity: {parity}")
@@ -274,6 +379,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": baseline,
         "parity": parity,
+        "forms": forms,
     }
     print(json.dumps(line), flush=True)
 

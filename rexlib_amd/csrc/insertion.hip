@@ -547,6 +547,8 @@ void dispatch_scatter(const insertion_params &p, hipStream_t stream)
 // ---------------------------------------------------------------------------
 
 constexpr int brick_edge = 16; // along x; the extents along y and z are parameters
+// Slots of 64 samples whose loads a wavefront issues together.
+constexpr int max_slots = 8;
 // LDS pitches of a brick: odd, so that lanes stepping along any axis fall in
 // different banks.
 constexpr int pitch_y = brick_edge + 1;
@@ -588,6 +590,12 @@ __device__ inline int wave_inclusive_scan(int value)
 		}
 	}
 	return value;
+}
+
+// A float held by lane `lane` (uniform), as a scalar.
+__device__ inline float lane_value(float v, int lane)
+{
+	return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
 // Keeps the compiler from moving LDS accesses across it: the taps of other
@@ -692,8 +700,9 @@ __device__ inline void add_sample(const insertion_params &p, const brick_box &bo
 
 // One slice, one sign, into a brick; the whole wavefront takes part.
 template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
-__device__ void insert_slice_sign(const insertion_params &p, const slice_entry &e, const float4 *slice,
-	const brick_box &box, const float *lower, const float *upper, int sign_index, float4 *acc)
+__device__ void insert_slice_sign(const insertion_params &p, const float *x_step, const float *y_step_f,
+	int last_shell, const float4 *slice, const brick_box &box, const float *lower, const float *upper, int sign_index,
+	float4 *acc)
 {
 	const int lane = threadIdx.x & 63;
 	const int n = p.n;
@@ -701,8 +710,8 @@ __device__ void insert_slice_sign(const insertion_params &p, const slice_entry &
 	const bool even = n % 2 == 0;
 	const int last_column = h - (even ? 2 : 1);
 	const int highest_row = even ? n / 2 - 1 : n / 2;
-	const float radius = static_cast<float>(e.last_shell) + 0.5f;
-	const int widest = min(last_column, e.last_shell);
+	const float radius = static_cast<float>(last_shell) + 0.5f;
+	const int widest = min(last_column, last_shell);
 	const int first_row = max(-highest_row, -widest);
 	const int last_row = min(highest_row, widest);
 	const float sign = sign_index == 0 ? 1.0f : -1.0f;
@@ -711,8 +720,8 @@ __device__ void insert_slice_sign(const insertion_params &p, const slice_entry &
 	float y_step[3];
 	for (int a = 0; a < 3; ++a)
 	{
-		step[a] = sign * e.x_step[a];
-		y_step[a] = sign * static_cast<float>(e.y_step[a]);
+		step[a] = sign * x_step[a];
+		y_step[a] = sign * y_step_f[a];
 	}
 	// bound_rows: the rows k_y whose samples k_x in [first, widest] can reach the box.
 	float low = static_cast<float>(first_row);
@@ -785,23 +794,52 @@ __device__ void insert_slice_sign(const insertion_params &p, const slice_entry &
 			}
 		}
 		// Four parity classes; a class holds rows of one parity of k_y and the
-		// k_x of one parity within them.
+		// k_x of one parity within them. Each class is packed densely over the
+		// lanes (a scan over the rows), in slots of 64 samples.
+		int class_begin[4];
+		int class_start[4];
+		int class_total[4];
+		int slot_end[4]; // running count of slots through each class
+		int slots = 0;
 		for (int cls = 0; cls < 4; ++cls)
 		{
 			const int py = cls >> 1;
 			const int px = cls & 1;
-			int class_begin = begin + ((begin - px) & 1); // first k_x >= begin of parity px
+			class_begin[cls] = begin + ((begin - px) & 1); // first k_x >= begin of parity px
 			int count = 0;
-			if (((ky - py) & 1) == 0 && class_begin < end)
+			if (((ky - py) & 1) == 0 && class_begin[cls] < end)
 			{
-				count = (end - class_begin + 1) / 2;
+				count = (end - class_begin[cls] + 1) / 2;
 			}
 			const int inclusive = wave_inclusive_scan(count);
-			const int start = inclusive - count;
-			const int total = __shfl(inclusive, 63, 64);
-			for (int base = 0; base < total; base += 64)
+			class_start[cls] = inclusive - count;
+			class_total[cls] = __shfl(inclusive, 63, 64);
+			slots += (class_total[cls] + 63) >> 6;
+			slot_end[cls] = slots;
+		}
+		// Gather, then scatter: the samples of up to max_slots slots are
+		// located and loaded first, so that their loads overlap, and added
+		// afterwards one slot at a time (one class per instruction).
+		for (int group = 0; group < slots; group += max_slots)
+		{
+			float4 q[max_slots];
+			int sample_x[max_slots];
+			int sample_y[max_slots];
+#pragma unroll
+			for (int k = 0; k < max_slots; ++k)
 			{
-				const int i = base + lane;
+				q[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+				sample_x[k] = 0;
+				sample_y[k] = 0;
+				const int g = group + k;
+				if (g >= slots)
+				{
+					continue;
+				}
+				const int cls = g < slot_end[0] ? 0 : (g < slot_end[1] ? 1 : (g < slot_end[2] ? 2 : 3));
+				const int first_slot = cls == 0 ? 0 : slot_end[cls - 1];
+				const int start = class_start[cls];
+				const int i = ((g - first_slot) << 6) + lane;
 				// The row holding sample i: the last lane whose start is at or below i.
 				int lo_r = 0;
 				int hi_r = 63;
@@ -818,31 +856,36 @@ __device__ void insert_slice_sign(const insertion_params &p, const slice_entry &
 						hi_r = mid - 1;
 					}
 				}
-				const int row_begin = __shfl(class_begin, lo_r, 64);
+				const int row_begin = __shfl(class_begin[cls], lo_r, 64);
 				const int row_start = __shfl(start, lo_r, 64);
-				bool active = i < total;
 				const int sky = rc + lo_r;
 				const int kx = row_begin + 2 * (i - row_start);
-				const int row = sky < 0 ? sky + n : sky;
-				float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-				if (active)
+				sample_x[k] = kx;
+				sample_y[k] = sky;
+				if (i < class_total[cls])
 				{
-					q = slice[row * h + kx];
-					active = !(q.x == 0.0f && q.y == 0.0f && q.z == 0.0f);
+					const int row = sky < 0 ? sky + n : sky;
+					q[k] = slice[row * h + kx];
 				}
-				if (!active)
+			}
+#pragma unroll
+			for (int k = 0; k < max_slots; ++k)
+			{
+				if (group + k >= slots)
 				{
-					// Stay in step with the other lanes; add nothing.
-					q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+					continue;
 				}
-				const float fx = static_cast<float>(kx);
-				const float fy = static_cast<float>(sky);
+				// Zero samples (not inserted) stay in step with the other lanes.
+				const bool active = !(q[k].x == 0.0f && q[k].y == 0.0f && q[k].z == 0.0f);
+				const float fx = static_cast<float>(sample_x[k]);
+				const float fy = static_cast<float>(sample_y[k]);
 				float c[3];
 				for (int a = 0; a < 3; ++a)
 				{
 					c[a] = active ? fy * y_step[a] + fx * step[a] : -1.0e6f;
 				}
-				add_sample<PROFILE, ODD, ATOMIC, EY, EZ>(p, box, acc, c, q.x, sign_index == 0 ? q.y : -q.y, q.z);
+				add_sample<PROFILE, ODD, ATOMIC, EY, EZ>(p, box, acc, c, q[k].x, sign_index == 0 ? q[k].y : -q[k].y,
+					q[k].z);
 			}
 		}
 	}
@@ -896,15 +939,23 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 			// Slices whose plane passes within reach of the brick, one per lane.
 			const int64_t j = window + lane;
 			bool near = false;
+			// The frame of this lane's slice, kept for the insertions below.
+			float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f, y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+			int last_shell = -1;
+			int lattice_free = 0;
 			if (j < share_end)
 			{
 				const slice_entry &e = p.entries[kept * p.map.reduced_count + chunk.begin + j];
+				last_shell = e.last_shell;
+				lattice_free = e.lattice_free;
 				if (e.last_shell >= 0)
 				{
-					const float x0 = e.x_step[0], x1 = e.x_step[1], x2 = e.x_step[2];
-					const float y0 = static_cast<float>(e.y_step[0]);
-					const float y1 = static_cast<float>(e.y_step[1]);
-					const float y2 = static_cast<float>(e.y_step[2]);
+					x0 = e.x_step[0];
+					x1 = e.x_step[1];
+					x2 = e.x_step[2];
+					y0 = static_cast<float>(e.y_step[0]);
+					y1 = static_cast<float>(e.y_step[1]);
+					y2 = static_cast<float>(e.y_step[2]);
 					const float nx = x1 * y2 - x2 * y1;
 					const float ny = x2 * y0 - x0 * y2;
 					const float nz = x0 * y1 - x1 * y0;
@@ -926,20 +977,23 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 				const int bit = __ffsll(static_cast<long long>(mask)) - 1;
 				mask &= mask - 1;
 				const int64_t js = window + bit;
-				const slice_entry &e = p.entries[kept * p.map.reduced_count + chunk.begin + js];
 				const float4 *slice = prepared + (kept * chunk.count + js) * elements;
-				const bool lattice_free = e.lattice_free != 0;
+				// The hit slice's frame, read from the lane that tested it.
+				const float x_step[3] = {lane_value(x0, bit), lane_value(x1, bit), lane_value(x2, bit)};
+				const float y_step[3] = {lane_value(y0, bit), lane_value(y1, bit), lane_value(y2, bit)};
+				const int shell = __builtin_amdgcn_readlane(last_shell, bit);
+				const bool exact = __builtin_amdgcn_readlane(lattice_free, bit) != 0;
 				for (int sign_index = 0; sign_index < 2; ++sign_index)
 				{
-					if (lattice_free)
+					if (exact)
 					{
-						insert_slice_sign<PROFILE, ODD, false, EY, EZ>(p, e, slice, box, lower, upper, sign_index,
-							acc);
+						insert_slice_sign<PROFILE, ODD, false, EY, EZ>(p, x_step, y_step, shell, slice, box, lower,
+							upper, sign_index, acc);
 					}
 					else
 					{
-						insert_slice_sign<PROFILE, ODD, true, EY, EZ>(p, e, slice, box, lower, upper, sign_index,
-							acc);
+						insert_slice_sign<PROFILE, ODD, true, EY, EZ>(p, x_step, y_step, shell, slice, box, lower,
+							upper, sign_index, acc);
 					}
 				}
 			}
