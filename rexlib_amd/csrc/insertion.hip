@@ -641,10 +641,83 @@ __global__ __launch_bounds__(256) void prepare_kernel(insertion_params p, slice_
 	}
 }
 
+// Box and triangle windows on an even N: per sample, the first tap's brick
+// index, its volume phase (-1)^(x+y+z) (a wrapped index keeps its parity)
+// and the in-brick bits of the taps; each tap is then a constant offset and a
+// constant sign flip. The sums are those of the general form below.
+template <int PROFILE, bool ATOMIC, int EY, int EZ>
+__device__ __forceinline__ void add_sample_even(const insertion_params &p, const brick_box &box, float4 *acc,
+	const float *c, float vr, float vi, float weight)
+{
+	constexpr int taps = window_taps<PROFILE>::taps;
+	constexpr int pitch_z = brick_pitch_z(EY);
+	float w[3][taps];
+	bool in[3][taps];
+	int g[3];
+	int local[3];
+#pragma unroll
+	for (int a = 0; a < 3; ++a)
+	{
+		g[a] = window_taps<PROFILE>::place(p, c[a], w[a]);
+		local[a] = g[a] - box.lo[a];
+#pragma unroll
+		for (int t = 0; t < taps; ++t)
+		{
+			in[a][t] = static_cast<unsigned>(local[a] + t) < static_cast<unsigned>(box.ext[a]);
+		}
+	}
+	const int base = local[0] + pitch_y * local[1] + pitch_z * local[2];
+	const bool odd = ((g[0] + g[1] + g[2]) & 1) != 0;
+	const float sr = odd ? -vr : vr;
+	const float si = odd ? -vi : vi;
+#pragma unroll
+	for (int tz = 0; tz < taps; ++tz)
+	{
+#pragma unroll
+		for (int ty = 0; ty < taps; ++ty)
+		{
+			const float wzy = w[2][tz] * w[1][ty];
+			const bool in_zy = in[2][tz] && in[1][ty];
+#pragma unroll
+			for (int tx = 0; tx < taps; ++tx)
+			{
+				const float ww = wzy * w[0][tx];
+				const bool flip = ((tx + ty + tz) & 1) != 0;
+				const float re = (flip ? -sr : sr) * ww;
+				const float im = (flip ? -si : si) * ww;
+				const int idx = base + tx + pitch_y * ty + pitch_z * tz;
+				if (in_zy && in[0][tx])
+				{
+					if (ATOMIC)
+					{
+						atomicAdd(&acc[idx].x, re);
+						atomicAdd(&acc[idx].y, im);
+						atomicAdd(&acc[idx].z, weight * ww);
+					}
+					else
+					{
+						float4 v = acc[idx];
+						v.x += re;
+						v.y += im;
+						v.z += weight * ww;
+						acc[idx] = v;
+					}
+				}
+				lds_order();
+			}
+		}
+	}
+}
+
 template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
 __device__ inline void add_sample(const insertion_params &p, const brick_box &box, float4 *acc, const float *c,
 	float vr, float vi, float weight)
 {
+	if constexpr (!ODD && PROFILE != REXLIB_HIP_KAISER_BESSEL)
+	{
+		add_sample_even<PROFILE, ATOMIC, EY, EZ>(p, box, acc, c, vr, vi, weight);
+		return;
+	}
 	constexpr int pitch_z = brick_pitch_z(EY);
 	float wx[window_taps<PROFILE>::taps];
 	float wy[window_taps<PROFILE>::taps];
@@ -698,65 +771,86 @@ __device__ inline void add_sample(const insertion_params &p, const brick_box &bo
 	}
 }
 
-// One slice, one sign, into a brick; the whole wavefront takes part.
+// bound_rows of one slice and sign: the rows [r0, r1) whose samples k_x in
+// [first, widest] can reach the box [lower, upper]; r0 >= r1 when none.
+// Evaluated per lane, one slice per lane.
+__device__ inline void bound_rows(const insertion_params &p, const float *x_step, const float *y_step_f,
+	int last_shell, const float *lower, const float *upper, int sign_index, int &r0, int &r1)
+{
+	r0 = 0;
+	r1 = 0;
+	const int n = p.n;
+	const int h = p.h;
+	const bool even = n % 2 == 0;
+	const int last_column = h - (even ? 2 : 1);
+	const int highest_row = even ? n / 2 - 1 : n / 2;
+	const int widest = min(last_column, last_shell);
+	const int first_row = max(-highest_row, -widest);
+	const int last_row = min(highest_row, widest);
+	const float sign = sign_index == 0 ? 1.0f : -1.0f;
+	const int first = sign_index;
+	if (last_shell < 0 || first > widest || first_row > last_row)
+	{
+		return;
+	}
+	float low = static_cast<float>(first_row);
+	float high = static_cast<float>(last_row);
+	bool none = false;
+	for (int a = 0; a < 3; ++a)
+	{
+		const float step = sign * x_step[a];
+		const float y_step = sign * y_step_f[a];
+		const float at_first = static_cast<float>(first) * step;
+		const float at_last = static_cast<float>(widest) * step;
+		const float lo = lower[a] - fmaxf(at_first, at_last);
+		const float hi = upper[a] - fminf(at_first, at_last);
+		if (y_step == 0.0f)
+		{
+			none = none || !(lo <= 0.0f && 0.0f <= hi);
+		}
+		else
+		{
+			const float l = lo / y_step;
+			const float u = hi / y_step;
+			low = fmaxf(low, fminf(l, u));
+			high = fminf(high, fmaxf(l, u));
+		}
+	}
+	if (none || low > high)
+	{
+		return;
+	}
+	// The box is already widened past the window, so rounding inwards keeps
+	// every sample that can reach it.
+	r0 = max(static_cast<int>(ceilf(low)), first_row);
+	r1 = min(static_cast<int>(floorf(high)) + 1, last_row + 1);
+}
+
+// One slice, one sign, into a brick, over the rows [r0, r1) that reach it;
+// the whole wavefront takes part.
 template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
 __device__ void insert_slice_sign(const insertion_params &p, const float *x_step, const float *y_step_f,
 	int last_shell, const float4 *slice, const brick_box &box, const float *lower, const float *upper, int sign_index,
-	float4 *acc)
+	int r0, int r1, float4 *acc)
 {
 	const int lane = threadIdx.x & 63;
 	const int n = p.n;
 	const int h = p.h;
 	const bool even = n % 2 == 0;
 	const int last_column = h - (even ? 2 : 1);
-	const int highest_row = even ? n / 2 - 1 : n / 2;
 	const float radius = static_cast<float>(last_shell) + 0.5f;
 	const int widest = min(last_column, last_shell);
-	const int first_row = max(-highest_row, -widest);
-	const int last_row = min(highest_row, widest);
 	const float sign = sign_index == 0 ? 1.0f : -1.0f;
 	const int first = sign_index;
 	float step[3];
 	float y_step[3];
+	float inv_step[3];
 	for (int a = 0; a < 3; ++a)
 	{
 		step[a] = sign * x_step[a];
 		y_step[a] = sign * y_step_f[a];
+		inv_step[a] = step[a] == 0.0f ? 0.0f : 1.0f / step[a];
 	}
-	// bound_rows: the rows k_y whose samples k_x in [first, widest] can reach the box.
-	float low = static_cast<float>(first_row);
-	float high = static_cast<float>(last_row);
-	if (first > widest || first_row > last_row)
-	{
-		return;
-	}
-	for (int a = 0; a < 3; ++a)
-	{
-		const float at_first = static_cast<float>(first) * step[a];
-		const float at_last = static_cast<float>(widest) * step[a];
-		const float lo = lower[a] - fmaxf(at_first, at_last);
-		const float hi = upper[a] - fminf(at_first, at_last);
-		if (y_step[a] == 0.0f)
-		{
-			if (!(lo <= 0.0f && 0.0f <= hi))
-			{
-				return;
-			}
-			continue;
-		}
-		const float l = lo / y_step[a];
-		const float u = hi / y_step[a];
-		low = fmaxf(low, fminf(l, u));
-		high = fminf(high, fmaxf(l, u));
-	}
-	if (low > high)
-	{
-		return;
-	}
-	// The box is already widened past the window, so rounding inwards keeps
-	// every sample that can reach it.
-	const int r0 = max(static_cast<int>(ceilf(low)), first_row);
-	const int r1 = min(static_cast<int>(floorf(high)) + 1, last_row + 1);
 	for (int rc = r0; rc < r1; rc += 64)
 	{
 		// clip_row: the samples of row k_y that can reach the box.
@@ -773,18 +867,23 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 				float lo_k = static_cast<float>(first);
 				float hi_k = static_cast<float>(last);
 				bool none = first > last;
-				for (int a = 0; a < 3 && !none; ++a)
+#pragma unroll
+				for (int a = 0; a < 3; ++a)
 				{
 					const float base = kyf * y_step[a];
 					if (step[a] == 0.0f)
 					{
-						none = !(lower[a] <= base && base <= upper[a]);
-						continue;
+						none = none || !(lower[a] <= base && base <= upper[a]);
 					}
-					const float l = (lower[a] - base) / step[a];
-					const float u = (upper[a] - base) / step[a];
-					lo_k = fmaxf(lo_k, fminf(l, u));
-					hi_k = fminf(hi_k, fmaxf(l, u));
+					else
+					{
+						// Reciprocal steps: the box is widened by 0.01 voxel,
+						// far beyond this rounding.
+						const float l = (lower[a] - base) * inv_step[a];
+						const float u = (upper[a] - base) * inv_step[a];
+						lo_k = fmaxf(lo_k, fminf(l, u));
+						hi_k = fminf(hi_k, fmaxf(l, u));
+					}
 				}
 				if (!none && lo_k <= hi_k)
 				{
@@ -800,20 +899,39 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 		int class_start[4];
 		int class_total[4];
 		int slot_end[4]; // running count of slots through each class
-		int slots = 0;
+		int count[4];
 		for (int cls = 0; cls < 4; ++cls)
 		{
 			const int py = cls >> 1;
 			const int px = cls & 1;
 			class_begin[cls] = begin + ((begin - px) & 1); // first k_x >= begin of parity px
-			int count = 0;
+			count[cls] = 0;
 			if (((ky - py) & 1) == 0 && class_begin[cls] < end)
 			{
-				count = (end - class_begin[cls] + 1) / 2;
+				count[cls] = (end - class_begin[cls] + 1) / 2;
 			}
-			const int inclusive = wave_inclusive_scan(count);
-			class_start[cls] = inclusive - count;
-			class_total[cls] = __shfl(inclusive, 63, 64);
+		}
+		// Two scans of two 16-bit counts each (a class holds fewer than
+		// 2^16 samples: at most 64 rows of at most a few hundred).
+		const int packed_a = count[0] | (count[1] << 16);
+		const int packed_b = count[2] | (count[3] << 16);
+		const int inclusive_a = wave_inclusive_scan(packed_a);
+		const int inclusive_b = wave_inclusive_scan(packed_b);
+		const int exclusive_a = inclusive_a - packed_a;
+		const int exclusive_b = inclusive_b - packed_b;
+		const int total_a = __shfl(inclusive_a, 63, 64);
+		const int total_b = __shfl(inclusive_b, 63, 64);
+		class_start[0] = exclusive_a & 0xffff;
+		class_start[1] = exclusive_a >> 16;
+		class_start[2] = exclusive_b & 0xffff;
+		class_start[3] = exclusive_b >> 16;
+		class_total[0] = total_a & 0xffff;
+		class_total[1] = total_a >> 16;
+		class_total[2] = total_b & 0xffff;
+		class_total[3] = total_b >> 16;
+		int slots = 0;
+		for (int cls = 0; cls < 4; ++cls)
+		{
 			slots += (class_total[cls] + 63) >> 6;
 			slot_end[cls] = slots;
 		}
@@ -971,6 +1089,16 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 						centre_norm <= spread * 1.001f + half_diagonal + 1e-3f;
 				}
 			}
+			// bound_rows of both signs, one slice per lane.
+			int r0_plus = 0, r1_plus = 0, r0_minus = 0, r1_minus = 0;
+			if (near)
+			{
+				const float xs[3] = {x0, x1, x2};
+				const float ys[3] = {y0, y1, y2};
+				bound_rows(p, xs, ys, last_shell, lower, upper, 0, r0_plus, r1_plus);
+				bound_rows(p, xs, ys, last_shell, lower, upper, 1, r0_minus, r1_minus);
+				near = r0_plus < r1_plus || r0_minus < r1_minus;
+			}
 			unsigned long long mask = __ballot(near);
 			while (mask != 0)
 			{
@@ -985,15 +1113,21 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 				const bool exact = __builtin_amdgcn_readlane(lattice_free, bit) != 0;
 				for (int sign_index = 0; sign_index < 2; ++sign_index)
 				{
+					const int r0 = __builtin_amdgcn_readlane(sign_index == 0 ? r0_plus : r0_minus, bit);
+					const int r1 = __builtin_amdgcn_readlane(sign_index == 0 ? r1_plus : r1_minus, bit);
+					if (r0 >= r1)
+					{
+						continue;
+					}
 					if (exact)
 					{
 						insert_slice_sign<PROFILE, ODD, false, EY, EZ>(p, x_step, y_step, shell, slice, box, lower,
-							upper, sign_index, acc);
+							upper, sign_index, r0, r1, acc);
 					}
 					else
 					{
 						insert_slice_sign<PROFILE, ODD, true, EY, EZ>(p, x_step, y_step, shell, slice, box, lower,
-							upper, sign_index, acc);
+							upper, sign_index, r0, r1, acc);
 					}
 				}
 			}
