@@ -526,6 +526,10 @@ void dispatch_scatter(const insertion_params &p, hipStream_t stream)
 // clip_row of src/backends/cpu/kernels/row_interval.cpp, restated per brick
 // as the reference restates them per tile (slice_insertion_functor.inl:391-440).
 //
+// A voxel takes 12 bytes of LDS (a 16 x 8 x 8 brick with its padding is
+// 13 KB), which lets 12 wavefronts share a CU: the adds of a tap wait on LDS
+// latency, and more wavefronts hide more of it.
+//
 // The samples are added by plain LDS read-modify-write, not LDS float
 // atomics (which retire at about a third of a lane per clock per CU on
 // gfx950, against eight for a read-modify-write). A lane owns one sample per
@@ -557,6 +561,14 @@ __host__ __device__ constexpr int brick_pitch_z(int extent_y)
 {
 	return pitch_y * extent_y + 1;
 }
+
+// A voxel of a brick in LDS: 12 bytes.
+struct brick_voxel
+{
+	float re;
+	float im;
+	float den;
+};
 
 struct brick_box
 {
@@ -646,7 +658,7 @@ __global__ __launch_bounds__(256) void prepare_kernel(insertion_params p, slice_
 // and the in-brick bits of the taps; each tap is then a constant offset and a
 // constant sign flip. The sums are those of the general form below.
 template <int PROFILE, bool ATOMIC, int EY, int EZ>
-__device__ __forceinline__ void add_sample_even(const insertion_params &p, const brick_box &box, float4 *acc,
+__device__ __forceinline__ void add_sample_even(const insertion_params &p, const brick_box &box, brick_voxel *acc,
 	const float *c, float vr, float vi, float weight)
 {
 	constexpr int taps = window_taps<PROFILE>::taps;
@@ -690,16 +702,16 @@ __device__ __forceinline__ void add_sample_even(const insertion_params &p, const
 				{
 					if (ATOMIC)
 					{
-						atomicAdd(&acc[idx].x, re);
-						atomicAdd(&acc[idx].y, im);
-						atomicAdd(&acc[idx].z, weight * ww);
+						atomicAdd(&acc[idx].re, re);
+						atomicAdd(&acc[idx].im, im);
+						atomicAdd(&acc[idx].den, weight * ww);
 					}
 					else
 					{
-						float4 v = acc[idx];
-						v.x += re;
-						v.y += im;
-						v.z += weight * ww;
+						brick_voxel v = acc[idx];
+						v.re += re;
+						v.im += im;
+						v.den += weight * ww;
 						acc[idx] = v;
 					}
 				}
@@ -710,7 +722,7 @@ __device__ __forceinline__ void add_sample_even(const insertion_params &p, const
 }
 
 template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
-__device__ inline void add_sample(const insertion_params &p, const brick_box &box, float4 *acc, const float *c,
+__device__ inline void add_sample(const insertion_params &p, const brick_box &box, brick_voxel *acc, const float *c,
 	float vr, float vi, float weight)
 {
 	if constexpr (!ODD && PROFILE != REXLIB_HIP_KAISER_BESSEL)
@@ -752,16 +764,16 @@ __device__ inline void add_sample(const insertion_params &p, const brick_box &bo
 				{
 					if (ATOMIC)
 					{
-						atomicAdd(&acc[idx].x, contribution.x);
-						atomicAdd(&acc[idx].y, contribution.y);
-						atomicAdd(&acc[idx].z, weight * w);
+						atomicAdd(&acc[idx].re, contribution.x);
+						atomicAdd(&acc[idx].im, contribution.y);
+						atomicAdd(&acc[idx].den, weight * w);
 					}
 					else
 					{
-						float4 v = acc[idx];
-						v.x += contribution.x;
-						v.y += contribution.y;
-						v.z += weight * w;
+						brick_voxel v = acc[idx];
+						v.re += contribution.x;
+						v.im += contribution.y;
+						v.den += weight * w;
 						acc[idx] = v;
 					}
 				}
@@ -831,7 +843,7 @@ __device__ inline void bound_rows(const insertion_params &p, const float *x_step
 template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
 __device__ void insert_slice_sign(const insertion_params &p, const float *x_step, const float *y_step_f,
 	int last_shell, const float4 *slice, const brick_box &box, const float *lower, const float *upper, int sign_index,
-	int r0, int r1, float4 *acc)
+	int r0, int r1, brick_voxel *acc)
 {
 	const int lane = threadIdx.x & 63;
 	const int n = p.n;
@@ -1015,7 +1027,7 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 {
 	constexpr int pitch_z = brick_pitch_z(EY);
 	constexpr int voxels_max = pitch_z * EZ;
-	__shared__ float4 acc[voxels_max];
+	__shared__ brick_voxel acc[voxels_max];
 
 	const int lane = threadIdx.x;
 	const int big = p.big;
@@ -1031,7 +1043,7 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 		const int voxels = box.ext[0] * box.ext[1] * box.ext[2];
 		for (int v = lane; v < voxels_max; v += 64)
 		{
-			acc[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			acc[v] = brick_voxel{0.0f, 0.0f, 0.0f};
 		}
 		lds_order();
 		// The box of positions that can reach the brick, widened beyond the
@@ -1144,10 +1156,10 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 			const int ly = rest % box.ext[1];
 			const int lz = rest / box.ext[1];
 			const int idx = lx + pitch_y * ly + pitch_z * lz;
-			const float4 sum = acc[idx];
-			const float re = sum.x;
-			const float im = sum.y;
-			const float d = sum.z;
+			const brick_voxel sum = acc[idx];
+			const float re = sum.re;
+			const float im = sum.im;
+			const float d = sum.den;
 			if (re == 0.0f && im == 0.0f && d == 0.0f)
 			{
 				continue;
