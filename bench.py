@@ -263,7 +263,7 @@ def main():
     import torch
 
     import rexlib_amd
-    from rexlib_amd import _native, em
+    from rexlib_amd import _native, em, tune
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.copy_calibration:
@@ -286,6 +286,9 @@ def main():
     window = em.GriddingWindow.linear()
     optics = em.CtfOptics(300.0, 2.7, 0.1)
     launches = [(s, min(s + L, args.images)) for s in range(0, args.images, L)]
+    # The autotuner's brick shape and share for this box size (rexlib_amd/tuning.json).
+    tuned = tune.apply(n, big)
+    log(f"brick tuning (extent_y, extent_z, share): {tuned}")
 
     def step():
         num.zero_()
@@ -375,6 +378,7 @@ def main():
             "workload": (f"plain FP32: {args.images} images {n}^2 -> {big}^3, launches of {L}, CTF + trilinear, "
                          "then quotient, irfft3, gridding correction"),
             "images": args.images, "n": n, "N": big, "launch": L, "window": "linear", "variant": "ctf",
+            "brick": {"extent_x": 16, "extent_y": tuned[0], "extent_z": tuned[1], "share": tuned[2]} if tuned else None,
         },
         "roofline": roofline,
         "cpu_baseline": baseline,

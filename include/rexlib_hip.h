@@ -94,8 +94,9 @@ void rexlib_hip_version(int32_t *major, int32_t *minor, int32_t *patch);
 int32_t rexlib_hip_device_count(void);
 
 /* Tuning of the LDS brick schedule of the insertions, process-wide: brick
- * extents along y and z (8 or 16; 16 along x), and the expected plane hits
- * per task that splits busy bricks (>= 16). 0 keeps the default.
+ * extents along y (8 or 16) and z (6, 8 or 16; 6 with 8 along y), 16 along
+ * x, and the expected plane hits per task that splits busy bricks (>= 16).
+ * 0 keeps the default (16 x 8 x 6, 256 hits).
  * No reference counterpart (the reference's CPU tuning is
  * slice_insertion_tuning, src/backends/cpu/builders/em/reconstruction/slice_insertion_tuning.cpp:10-13);
  * set by the autotuner (rexlib_amd/tune.py). */

@@ -526,8 +526,8 @@ void dispatch_scatter(const insertion_params &p, hipStream_t stream)
 // clip_row of src/backends/cpu/kernels/row_interval.cpp, restated per brick
 // as the reference restates them per tile (slice_insertion_functor.inl:391-440).
 //
-// A voxel takes 12 bytes of LDS (a 16 x 8 x 8 brick with its padding is
-// 13 KB), which lets 12 wavefronts share a CU: the adds of a tap wait on LDS
+// A voxel takes 12 bytes of LDS (a 16 x 8 x 6 brick with its padding is
+// 10 KB), which lets 16 wavefronts share a CU: the adds of a tap wait on LDS
 // latency, and more wavefronts hide more of it.
 //
 // The samples are added by plain LDS read-modify-write, not LDS float
@@ -552,7 +552,7 @@ void dispatch_scatter(const insertion_params &p, hipStream_t stream)
 
 constexpr int brick_edge = 16; // along x; the extents along y and z are parameters
 // Slots of 64 samples whose loads a wavefront issues together.
-constexpr int max_slots = 8;
+constexpr int max_slots = 4;
 // LDS pitches of a brick: odd, so that lanes stepping along any axis fall in
 // different banks.
 constexpr int pitch_y = brick_edge + 1;
@@ -1195,6 +1195,8 @@ std::atomic<int> tuned_ey{0};
 std::atomic<int> tuned_ez{0};
 std::atomic<int> tuned_share{0};
 
+// Shapes: 16 x 8 x 6 (the default: 10 KB of LDS, 16 wavefronts per CU),
+// 16 x 8 x 8, 16 x 8 x 16, 16 x 16 x 8 and 16 x 16 x 16.
 int brick_ey_choice()
 {
 	const int v = tuned_ey.load();
@@ -1204,7 +1206,12 @@ int brick_ey_choice()
 int brick_ez_choice()
 {
 	const int v = tuned_ez.load();
-	return v == 16 ? 16 : 8;
+	if (v == 16 || v == 8)
+	{
+		return v;
+	}
+	// 6 needs a y extent of 8.
+	return brick_ey_choice() == 8 ? 6 : 8;
 }
 
 template <int PROFILE, int EY, int EZ>
@@ -1228,7 +1235,11 @@ void launch_brick(const insertion_params &p, slice_chunk chunk, const float4 *pr
 {
 	const int ey = brick_ey_choice();
 	const int ez = brick_ez_choice();
-	if (ey == 8 && ez == 8)
+	if (ey == 8 && ez == 6)
+	{
+		launch_brick_shape<PROFILE, 8, 6>(p, chunk, prepared, tasks, task_count, reach, stream);
+	}
+	else if (ey == 8 && ez == 8)
 	{
 		launch_brick_shape<PROFILE, 8, 8>(p, chunk, prepared, tasks, task_count, reach, stream);
 	}

@@ -2,7 +2,7 @@
 
 For a workload (slice extent n, volume extent N, slices per launch L) it times
 the insertion on the device for every brick shape (extents along y and z;
-the extent along x is 16) and share target (expected plane hits per task), and
+the extent along x is 16; 6 along z goes with 8 along y) and share target (expected plane hits per task), and
 records the fastest in a JSON table keyed by "n,N". ``apply(n, N)`` sets the
 recorded choice in the library before an insertion.
 
@@ -22,7 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
 EXTENTS_Y = (8, 16)
-EXTENTS_Z = (8, 16)
+EXTENTS_Z = (6, 8, 16)
 SHARES = (64, 128, 256, 512, 1024)
 
 
@@ -107,6 +107,8 @@ def tune(n: int, big: int, launch: int, path: str = TABLE, verbose: bool = True)
     results = []
     for ey in EXTENTS_Y:
         for ez in EXTENTS_Z:
+            if ez == 6 and ey != 8:
+                continue  # 6 along z is built for 8 along y only
             for share in SHARES:
                 set_tuning(ey, ez, share)
                 ms = time_insertion(n, big, slices, transforms, ctf)
