@@ -42,7 +42,7 @@ def algorithmic_bytes_per_image(n, big, launch, pixel_bytes=4):
     return n * n * pixel_bytes + 64 + 24.0 * big * big * (big // 2 + 1) / launch
 
 
-def traffic_per_launch(args):
+def traffic_per_launch(args, tuned):
     """HBM bytes per brick-kernel launch of this workload, from the committed
     rocprofv3 FETCH_SIZE and WRITE_SIZE passes (KiB per dispatch, scaled by the
     factor a copy kernel of known size gives), or None when absent."""
@@ -53,6 +53,9 @@ def traffic_per_launch(args):
         record = json.load(f)
     if record.get("n") != args.n or record.get("N") != args.big or record.get("launch") != args.launch:
         return None
+    brick = record.get("brick", {})
+    if tuned is None or (brick.get("extent_y"), brick.get("extent_z"), brick.get("share")) != tuple(tuned):
+        return None  # measured for another brick shape
     return record["bytes_per_launch"]
 
 
@@ -332,7 +335,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS if achieved else None,
-        "traffic": traffic_per_launch(args),
+        "traffic": traffic_per_launch(args, tuned),
         "launch_ms": avg_ms,
         "launches": brick_launches,
         "images_per_launch": images_per_launch,
