@@ -20,6 +20,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 namespace rexlib_hip
@@ -216,6 +217,77 @@ __device__ float2 *block_fft(float2 *a, float2 *b, int lines, int line_stride, c
 	return a;
 }
 
+// The same transform for a power-of-two length N known at compile time:
+// radix 4 while it divides, then one radix 2 (the order make_fft_plan gives),
+// with every index computed by shifts and masks.
+template <int N, int R, int NS>
+__device__ inline void stage_fixed(const float2 *a, float2 *b, int lines, int line_stride, const float2 *tw,
+	float sign)
+{
+	constexpr int m = N / R;
+	constexpr int twiddle_step = N / (NS * R);
+	for (int t = threadIdx.x; t < lines * m; t += blockDim.x)
+	{
+		const int line = t / m;
+		const int j = t % m;
+		const int k = j % NS;
+		const float2 *src = a + line * line_stride;
+		float2 v[R];
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+		{
+			v[r] = src[j + r * m];
+		}
+#pragma unroll
+		for (int r = 1; r < R; ++r)
+		{
+			float2 w = tw[k * r * twiddle_step];
+			w.y *= -sign; // the table holds exp(-2 pi i m / n)
+			v[r] = cmul(v[r], w);
+		}
+		dft<R>(v, sign);
+		const int d = (j / NS) * NS * R + k;
+		float2 *dst = b + line * line_stride;
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+		{
+			dst[d + r * NS] = v[r];
+		}
+	}
+}
+
+template <int N, int NS = 1>
+__device__ inline float2 *block_fft_fixed(float2 *a, float2 *b, int lines, int line_stride, const float2 *tw,
+	float sign)
+{
+	if constexpr (NS >= N)
+	{
+		return a;
+	}
+	else
+	{
+		constexpr int R = (N / NS) % 4 == 0 ? 4 : 2;
+		stage_fixed<N, R, NS>(a, b, lines, line_stride, tw, sign);
+		__syncthreads();
+		return block_fft_fixed<N, NS * R>(b, a, lines, line_stride, tw, sign);
+	}
+}
+
+// The fixed form for N > 0, the planned one otherwise.
+template <int N>
+__device__ inline float2 *transform(float2 *a, float2 *b, int lines, int line_stride, const fft_plan &plan,
+	const float2 *tw, float sign)
+{
+	if constexpr (N > 0)
+	{
+		return block_fft_fixed<N>(a, b, lines, line_stride, tw, sign);
+	}
+	else
+	{
+		return block_fft(a, b, lines, line_stride, plan, tw, sign);
+	}
+}
+
 template <int KIND>
 __device__ inline float load_real(const void *base, int64_t index);
 
@@ -241,17 +313,22 @@ __device__ inline float load_real<real_f16>(const void *base, int64_t index)
 // are zero-padded to the transform length n with their origin m/2 moved to
 // n/2. `rows` counts the padded rows, n per image, in line set `in` of images
 // (middle = image row, outer = image); the output rows follow line set `out`.
-template <int KIND>
+template <int KIND, int N>
 __global__ void rfft_rows_kernel(const void *in, line_set in_rows, int m, const float *scales,
 	int64_t scale_stride, float2 *out, line_set out_rows, fft_plan plan, const float2 *tw, int pairs)
 {
 	extern __shared__ float2 lds[];
-	const int n = plan.n;
+	const int n = N > 0 ? N : plan.n;
 	const int h = n / 2 + 1;
 	const int stride = n + 1;
 	float2 *a = lds;
 	float2 *b = lds + pairs * stride;
 	float2 *twl = lds + 2 * pairs * stride;
+	// Per padded row of the group: its first input element (-1: a zero row
+	// of the padding or past the end) and its first output element.
+	int64_t *row_in = reinterpret_cast<int64_t *>(twl + n);
+	int64_t *row_out = row_in + 2 * pairs;
+	float *row_scale = reinterpret_cast<float *>(row_out + 2 * pairs);
 	for (int i = threadIdx.x; i < n; i += blockDim.x)
 	{
 		twl[i] = tw[i];
@@ -261,62 +338,67 @@ __global__ void rfft_rows_kernel(const void *in, line_set in_rows, int m, const 
 	for (int64_t first = blockIdx.x * static_cast<int64_t>(2 * pairs); first < total;
 		first += static_cast<int64_t>(gridDim.x) * 2 * pairs)
 	{
+		for (int q = threadIdx.x; q < 2 * pairs; q += blockDim.x)
+		{
+			const int64_t row = first + q;
+			int64_t src = -1;
+			int64_t dst = -1;
+			float scale = 1.0f;
+			if (row < total)
+			{
+				const int64_t image = row / n;
+				const int r = static_cast<int>(row - image * n) - offset;
+				if (r >= 0 && r < m)
+				{
+					src = line_offset(in_rows, image * m + r);
+					if (scales != nullptr)
+					{
+						scale = scales[image * scale_stride];
+					}
+				}
+				dst = line_offset(out_rows, row);
+			}
+			row_in[q] = src;
+			row_out[q] = dst;
+			row_scale[q] = scale;
+		}
+		__syncthreads();
 		for (int t = threadIdx.x; t < pairs * n; t += blockDim.x)
 		{
 			const int p = t / n;
 			const int i = t - p * n;
-			float re = 0.0f;
-			float im = 0.0f;
+			const int c = i - offset;
+			float value[2] = {0.0f, 0.0f};
 			for (int half = 0; half < 2; ++half)
 			{
-				const int64_t row = first + 2 * p + half;
-				float value = 0.0f;
-				if (row < total)
+				const int64_t src = row_in[2 * p + half];
+				if (src >= 0 && c >= 0 && c < m)
 				{
-					const int64_t image = row / n;
-					const int r = static_cast<int>(row - image * n) - offset;
-					const int c = i - offset;
-					if (r >= 0 && r < m && c >= 0 && c < m)
-					{
-						const int64_t src = line_offset(in_rows, image * m + r) + c * in_rows.s_elem;
-						value = load_real<KIND>(in, src);
-						if (scales != nullptr)
-						{
-							value *= scales[image * scale_stride];
-						}
-					}
-				}
-				if (half == 0)
-				{
-					re = value;
-				}
-				else
-				{
-					im = value;
+					value[half] = load_real<KIND>(in, src + c * in_rows.s_elem) * row_scale[2 * p + half];
 				}
 			}
-			a[p * stride + i] = make_float2(re, im);
+			a[p * stride + i] = make_float2(value[0], value[1]);
 		}
 		__syncthreads();
-		const float2 *z = block_fft(a, b, pairs, stride, plan, twl, -1.0f);
+		const float2 *z = transform<N>(a, b, pairs, stride, plan, twl, -1.0f);
 		for (int t = threadIdx.x; t < pairs * h; t += blockDim.x)
 		{
 			const int p = t / h;
 			const int k = t - p * h;
 			const float2 zk = z[p * stride + k];
 			const float2 zm = cconj(z[p * stride + (n - k) % n]);
-			const int64_t row0 = first + 2 * p;
-			if (row0 < total)
+			const int64_t dst0 = row_out[2 * p];
+			const int64_t dst1 = row_out[2 * p + 1];
+			if (dst0 >= 0)
 			{
 				const float2 sum = cadd(zk, zm);
-				out[line_offset(out_rows, row0) + k * out_rows.s_elem] = make_float2(0.5f * sum.x, 0.5f * sum.y);
+				out[dst0 + k * out_rows.s_elem] = make_float2(0.5f * sum.x, 0.5f * sum.y);
 			}
-			if (row0 + 1 < total)
+			if (dst1 >= 0)
 			{
 				// (zk - zm) / (2 i)
 				const float2 diff = csub(zk, zm);
-				out[line_offset(out_rows, row0 + 1) + k * out_rows.s_elem] =
-					make_float2(0.5f * diff.y, -0.5f * diff.x);
+				out[dst1 + k * out_rows.s_elem] = make_float2(0.5f * diff.y, -0.5f * diff.x);
 			}
 		}
 		__syncthreads();
@@ -325,15 +407,18 @@ __global__ void rfft_rows_kernel(const void *in, line_set in_rows, int m, const 
 
 // Complex lines from `src` to `dst` (may be the same memory), `group` lines
 // per step; consecutive lines of a group are loaded by consecutive threads.
+template <int N>
 __global__ void fft_lines_kernel(const float2 *src, line_set src_lines, float2 *dst, line_set dst_lines,
 	fft_plan plan, const float2 *tw, float sign, float scale, int group)
 {
 	extern __shared__ float2 lds[];
-	const int n = plan.n;
+	const int n = N > 0 ? N : plan.n;
 	const int stride = n + 1;
 	float2 *a = lds;
 	float2 *b = lds + group * stride;
 	float2 *twl = lds + 2 * group * stride;
+	int64_t *line_src = reinterpret_cast<int64_t *>(twl + n);
+	int64_t *line_dst = line_src + group;
 	for (int i = threadIdx.x; i < n; i += blockDim.x)
 	{
 		twl[i] = tw[i];
@@ -343,17 +428,23 @@ __global__ void fft_lines_kernel(const float2 *src, line_set src_lines, float2 *
 		first += static_cast<int64_t>(gridDim.x) * group)
 	{
 		const int lines = static_cast<int>(total - first < group ? total - first : group);
+		for (int q = threadIdx.x; q < lines; q += blockDim.x)
+		{
+			line_src[q] = line_offset(src_lines, first + q);
+			line_dst[q] = line_offset(dst_lines, first + q);
+		}
+		__syncthreads();
 		for (int t = threadIdx.x; t < group * n; t += blockDim.x)
 		{
 			const int e = t / group;
 			const int l = t - e * group;
 			if (l < lines)
 			{
-				a[l * stride + e] = src[line_offset(src_lines, first + l) + e * src_lines.s_elem];
+				a[l * stride + e] = src[line_src[l] + e * src_lines.s_elem];
 			}
 		}
 		__syncthreads();
-		const float2 *z = block_fft(a, b, lines, stride, plan, twl, sign);
+		const float2 *z = transform<N>(a, b, lines, stride, plan, twl, sign);
 		for (int t = threadIdx.x; t < group * n; t += blockDim.x)
 		{
 			const int e = t / group;
@@ -361,7 +452,7 @@ __global__ void fft_lines_kernel(const float2 *src, line_set src_lines, float2 *
 			if (l < lines)
 			{
 				const float2 v = z[l * stride + e];
-				dst[line_offset(dst_lines, first + l) + e * dst_lines.s_elem] = make_float2(v.x * scale, v.y * scale);
+				dst[line_dst[l] + e * dst_lines.s_elem] = make_float2(v.x * scale, v.y * scale);
 			}
 		}
 		__syncthreads();
@@ -369,16 +460,19 @@ __global__ void fft_lines_kernel(const float2 *src, line_set src_lines, float2 *
 }
 
 // Half-spectrum rows, two at a time, to real rows, scaled.
+template <int N>
 __global__ void irfft_rows_kernel(const float2 *src, line_set src_rows, float *dst, line_set dst_rows,
 	fft_plan plan, const float2 *tw, float scale, int pairs)
 {
 	extern __shared__ float2 lds[];
-	const int n = plan.n;
+	const int n = N > 0 ? N : plan.n;
 	const int h = n / 2 + 1;
 	const int stride = n + 1;
 	float2 *a = lds;
 	float2 *b = lds + pairs * stride;
 	float2 *twl = lds + 2 * pairs * stride;
+	int64_t *row_src = reinterpret_cast<int64_t *>(twl + n);
+	int64_t *row_dst = row_src + 2 * pairs;
 	for (int i = threadIdx.x; i < n; i += blockDim.x)
 	{
 		twl[i] = tw[i];
@@ -387,6 +481,13 @@ __global__ void irfft_rows_kernel(const float2 *src, line_set src_rows, float *d
 	for (int64_t first = blockIdx.x * static_cast<int64_t>(2 * pairs); first < total;
 		first += static_cast<int64_t>(gridDim.x) * 2 * pairs)
 	{
+		for (int q = threadIdx.x; q < 2 * pairs; q += blockDim.x)
+		{
+			const int64_t row = first + q;
+			row_src[q] = row < total ? line_offset(src_rows, row) : -1;
+			row_dst[q] = row < total ? line_offset(dst_rows, row) : -1;
+		}
+		__syncthreads();
 		for (int t = threadIdx.x; t < pairs * n; t += blockDim.x)
 		{
 			const int p = t / n;
@@ -396,14 +497,15 @@ __global__ void irfft_rows_kernel(const float2 *src, line_set src_rows, float *d
 			const bool self_conjugate = kk == 0 || (n % 2 == 0 && kk == n / 2);
 			float2 va = make_float2(0.0f, 0.0f);
 			float2 vb = make_float2(0.0f, 0.0f);
-			const int64_t row0 = first + 2 * p;
-			if (row0 < total)
+			const int64_t src0 = row_src[2 * p];
+			const int64_t src1 = row_src[2 * p + 1];
+			if (src0 >= 0)
 			{
-				va = src[line_offset(src_rows, row0) + kk * src_rows.s_elem];
+				va = src[src0 + kk * src_rows.s_elem];
 			}
-			if (row0 + 1 < total)
+			if (src1 >= 0)
 			{
-				vb = src[line_offset(src_rows, row0 + 1) + kk * src_rows.s_elem];
+				vb = src[src1 + kk * src_rows.s_elem];
 			}
 			if (self_conjugate)
 			{
@@ -419,20 +521,21 @@ __global__ void irfft_rows_kernel(const float2 *src, line_set src_rows, float *d
 			a[p * stride + k] = make_float2(va.x - vb.y, va.y + vb.x);
 		}
 		__syncthreads();
-		const float2 *z = block_fft(a, b, pairs, stride, plan, twl, 1.0f);
+		const float2 *z = transform<N>(a, b, pairs, stride, plan, twl, 1.0f);
 		for (int t = threadIdx.x; t < pairs * n; t += blockDim.x)
 		{
 			const int p = t / n;
 			const int i = t - p * n;
 			const float2 v = z[p * stride + i];
-			const int64_t row0 = first + 2 * p;
-			if (row0 < total)
+			const int64_t dst0 = row_dst[2 * p];
+			const int64_t dst1 = row_dst[2 * p + 1];
+			if (dst0 >= 0)
 			{
-				dst[line_offset(dst_rows, row0) + i * dst_rows.s_elem] = v.x * scale;
+				dst[dst0 + i * dst_rows.s_elem] = v.x * scale;
 			}
-			if (row0 + 1 < total)
+			if (dst1 >= 0)
 			{
-				dst[line_offset(dst_rows, row0 + 1) + i * dst_rows.s_elem] = v.y * scale;
+				dst[dst1 + i * dst_rows.s_elem] = v.y * scale;
 			}
 		}
 		__syncthreads();
@@ -453,29 +556,49 @@ int lines_per_group(int n)
 
 } // anonymous namespace
 
+// Calls f.template operator()<N>() with N the length when it is a power of
+// two from 64 to 512, else 0 (the planned transform).
+template <typename F>
+void with_fixed_length(int64_t n, F &&f)
+{
+	switch (n)
+	{
+	case 64: f(std::integral_constant<int, 64>()); break;
+	case 128: f(std::integral_constant<int, 128>()); break;
+	case 256: f(std::integral_constant<int, 256>()); break;
+	case 512: f(std::integral_constant<int, 512>()); break;
+	default: f(std::integral_constant<int, 0>()); break;
+	}
+}
+
 void launch_rfft_rows(int kind, const void *in, const line_set &in_rows, int m, const float *scales,
 	int64_t scale_stride, float2 *out, const line_set &out_rows, int64_t n, hipStream_t stream)
 {
 	const fft_plan plan = make_fft_plan(n);
 	const float2 *tw = twiddles(n);
 	const int pairs = std::max(1, lines_per_group(static_cast<int>(n)) / 2);
-	const std::size_t shared = sizeof(float2) * (2 * pairs * (n + 1) + n);
+	// Two buffers, the twiddles, then per row two offsets and a scale.
+	const std::size_t shared = sizeof(float2) * (2 * pairs * (n + 1) + n) + 2 * pairs * (2 * sizeof(int64_t) + 4);
 	const unsigned grid = grid_for(out_rows.count, 2 * pairs);
-	switch (kind)
-	{
-	case real_f32:
-		rfft_rows_kernel<real_f32><<<grid, 256, shared, stream>>>(in, in_rows, m, scales, scale_stride, out,
-			out_rows, plan, tw, pairs);
-		break;
-	case real_bf16:
-		rfft_rows_kernel<real_bf16><<<grid, 256, shared, stream>>>(in, in_rows, m, scales, scale_stride, out,
-			out_rows, plan, tw, pairs);
-		break;
-	default:
-		rfft_rows_kernel<real_f16><<<grid, 256, shared, stream>>>(in, in_rows, m, scales, scale_stride, out,
-			out_rows, plan, tw, pairs);
-		break;
-	}
+	with_fixed_length(n, [&](auto length)
+		{
+			constexpr int N = decltype(length)::value;
+			switch (kind)
+			{
+			case real_f32:
+				rfft_rows_kernel<real_f32, N><<<grid, 256, shared, stream>>>(in, in_rows, m, scales, scale_stride,
+					out, out_rows, plan, tw, pairs);
+				break;
+			case real_bf16:
+				rfft_rows_kernel<real_bf16, N><<<grid, 256, shared, stream>>>(in, in_rows, m, scales, scale_stride,
+					out, out_rows, plan, tw, pairs);
+				break;
+			default:
+				rfft_rows_kernel<real_f16, N><<<grid, 256, shared, stream>>>(in, in_rows, m, scales, scale_stride,
+					out, out_rows, plan, tw, pairs);
+				break;
+			}
+		});
 	REXLIB_HIP_CHECK(hipGetLastError());
 }
 
@@ -485,9 +608,14 @@ void launch_fft_lines(const float2 *src, const line_set &src_lines, float2 *dst,
 	const fft_plan plan = make_fft_plan(n);
 	const float2 *tw = twiddles(n);
 	const int group = lines_per_group(static_cast<int>(n));
-	const std::size_t shared = sizeof(float2) * (2 * group * (n + 1) + n);
+	const std::size_t shared = sizeof(float2) * (2 * group * (n + 1) + n) + 2 * group * sizeof(int64_t);
 	const unsigned grid = grid_for(src_lines.count, group);
-	fft_lines_kernel<<<grid, 256, shared, stream>>>(src, src_lines, dst, dst_lines, plan, tw, sign, scale, group);
+	with_fixed_length(n, [&](auto length)
+		{
+			constexpr int N = decltype(length)::value;
+			fft_lines_kernel<N><<<grid, 256, shared, stream>>>(src, src_lines, dst, dst_lines, plan, tw, sign, scale,
+				group);
+		});
 	REXLIB_HIP_CHECK(hipGetLastError());
 }
 
@@ -497,9 +625,13 @@ void launch_irfft_rows(const float2 *src, const line_set &src_rows, float *dst, 
 	const fft_plan plan = make_fft_plan(n);
 	const float2 *tw = twiddles(n);
 	const int pairs = std::max(1, lines_per_group(static_cast<int>(n)) / 2);
-	const std::size_t shared = sizeof(float2) * (2 * pairs * (n + 1) + n);
+	const std::size_t shared = sizeof(float2) * (2 * pairs * (n + 1) + n) + 2 * pairs * 2 * sizeof(int64_t);
 	const unsigned grid = grid_for(src_rows.count, 2 * pairs);
-	irfft_rows_kernel<<<grid, 256, shared, stream>>>(src, src_rows, dst, dst_rows, plan, tw, scale, pairs);
+	with_fixed_length(n, [&](auto length)
+		{
+			constexpr int N = decltype(length)::value;
+			irfft_rows_kernel<N><<<grid, 256, shared, stream>>>(src, src_rows, dst, dst_rows, plan, tw, scale, pairs);
+		});
 	REXLIB_HIP_CHECK(hipGetLastError());
 }
 
