@@ -604,6 +604,19 @@ __device__ inline int wave_inclusive_scan(int value)
 	return value;
 }
 
+// Prefix maximum over the 64 lanes (lane i gets the maximum of lanes 0..i),
+// by DPP row shifts within rows of 16 and row broadcasts across them.
+__device__ __forceinline__ int wave_prefix_max(int v)
+{
+	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xf, 0xf, false)); // row_shr:1
+	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xf, 0xf, false)); // row_shr:2
+	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xf, 0xf, false)); // row_shr:4
+	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xf, 0xf, false)); // row_shr:8
+	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xa, 0xf, false)); // row_bcast:15
+	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xc, 0xf, false)); // row_bcast:31
+	return v;
+}
+
 // A float held by lane `lane` (uniform), as a scalar.
 __device__ inline float lane_value(float v, int lane)
 {
@@ -843,7 +856,7 @@ __device__ inline void bound_rows(const insertion_params &p, const float *x_step
 template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
 __device__ void insert_slice_sign(const insertion_params &p, const float *x_step, const float *y_step_f,
 	int last_shell, const float4 *slice, const brick_box &box, const float *lower, const float *upper, int sign_index,
-	int r0, int r1, brick_voxel *acc)
+	int r0, int r1, brick_voxel *acc, int *row_table)
 {
 	const int lane = threadIdx.x & 63;
 	const int n = p.n;
@@ -947,6 +960,70 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 			slots += (class_total[cls] + 63) >> 6;
 			slot_end[cls] = slots;
 		}
+		// The common case, every class within one slot: class c is slot c,
+		// and the row of each sample comes from the wave's row table (each
+		// non-empty row records its lane at its first sample, then a prefix
+		// maximum over the lanes) rather than from a search. Every load index
+		// is checked against the slice before use.
+		if (class_total[0] <= 64 && class_total[1] <= 64 && class_total[2] <= 64 && class_total[3] <= 64)
+		{
+			float4 q[4];
+			int sample_x[4];
+			int sample_y[4];
+			bool valid[4];
+#pragma unroll
+			for (int c = 0; c < 4; ++c)
+			{
+				q[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+				sample_x[c] = 0;
+				sample_y[c] = 0;
+				valid[c] = false;
+				if (class_total[c] == 0)
+				{
+					continue;
+				}
+				row_table[lane] = -1;
+				lds_order();
+				if (count[c] > 0)
+				{
+					atomicMax(&row_table[class_start[c] & 63], lane);
+				}
+				lds_order();
+				const int r = wave_prefix_max(row_table[lane]);
+				lds_order();
+				const int rr = r & 63;
+				const int row_begin = __shfl(class_begin[c], rr, 64);
+				const int row_start = __shfl(class_start[c], rr, 64);
+				const int sky = rc + rr;
+				const int kx = row_begin + 2 * (lane - row_start);
+				const int row = sky < 0 ? sky + n : sky;
+				sample_x[c] = kx;
+				sample_y[c] = sky;
+				valid[c] = lane < class_total[c] && r >= 0 && row >= 0 && row < n && kx >= 0 && kx < h;
+				q[c] = slice[valid[c] ? row * h + kx : 0];
+			}
+#pragma unroll
+			for (int c = 0; c < 4; ++c)
+			{
+				if (class_total[c] == 0)
+				{
+					continue;
+				}
+				// Lanes without a sample, and zero samples (not inserted), stay
+				// in step with the other lanes and add nothing.
+				const bool active = valid[c] && !(q[c].x == 0.0f && q[c].y == 0.0f && q[c].z == 0.0f);
+				const float fx = static_cast<float>(sample_x[c]);
+				const float fy = static_cast<float>(sample_y[c]);
+				float position[3];
+				for (int a = 0; a < 3; ++a)
+				{
+					position[a] = active ? fy * y_step[a] + fx * step[a] : -1.0e6f;
+				}
+				add_sample<PROFILE, ODD, ATOMIC, EY, EZ>(p, box, acc, position, q[c].x,
+					sign_index == 0 ? q[c].y : -q[c].y, q[c].z);
+			}
+			continue;
+		}
 		// Gather, then scatter: the samples of up to max_slots slots are
 		// located and loaded first, so that their loads overlap, and added
 		// afterwards one slot at a time (one class per instruction).
@@ -1028,6 +1105,8 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 	constexpr int pitch_z = brick_pitch_z(EY);
 	constexpr int voxels_max = pitch_z * EZ;
 	__shared__ brick_voxel acc[voxels_max];
+	// Per sample slot: the row of each sample (see insert_slice_sign).
+	__shared__ int row_table[64];
 
 	const int lane = threadIdx.x;
 	const int big = p.big;
@@ -1134,12 +1213,12 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 					if (exact)
 					{
 						insert_slice_sign<PROFILE, ODD, false, EY, EZ>(p, x_step, y_step, shell, slice, box, lower,
-							upper, sign_index, r0, r1, acc);
+							upper, sign_index, r0, r1, acc, row_table);
 					}
 					else
 					{
 						insert_slice_sign<PROFILE, ODD, true, EY, EZ>(p, x_step, y_step, shell, slice, box, lower,
-							upper, sign_index, r0, r1, acc);
+							upper, sign_index, r0, r1, acc, row_table);
 					}
 				}
 			}
