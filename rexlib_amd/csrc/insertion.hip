@@ -399,14 +399,17 @@ __device__ inline float2 volume_phase(const insertion_params &p, float2 value, i
 template <int PROFILE, int VARIANT, bool ODD>
 __global__ __launch_bounds__(256) void scatter_kernel(insertion_params p)
 {
-	const int64_t elements = static_cast<int64_t>(p.n) * p.h;
-	const int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-	if (idx >= elements)
+	// A slice holds n (n/2 + 1) < 2^31 elements: 32-bit indices (a 64-bit
+	// division per element would cost more than the sample itself).
+	const int elements32 = p.n * p.h;
+	const int64_t elements = elements32;
+	const int idx = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+	if (idx >= elements32)
 	{
 		return;
 	}
-	const int row = static_cast<int>(idx / p.h);
-	const int column = static_cast<int>(idx % p.h);
+	const int row = static_cast<int>(static_cast<unsigned>(idx) / static_cast<unsigned>(p.h));
+	const int column = idx - row * p.h;
 	const int big = p.big;
 	const int half_extent = big / 2;
 	for (int64_t s = blockIdx.y; s < p.slice_count; s += gridDim.y)
@@ -641,14 +644,17 @@ struct slice_chunk
 template <int VARIANT>
 __global__ __launch_bounds__(256) void prepare_kernel(insertion_params p, slice_chunk chunk, float4 *prepared)
 {
-	const int64_t elements = static_cast<int64_t>(p.n) * p.h;
-	const int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-	if (idx >= elements)
+	// A slice holds n (n/2 + 1) < 2^31 elements: 32-bit indices (a 64-bit
+	// division per element would cost more than the sample itself).
+	const int elements32 = p.n * p.h;
+	const int64_t elements = elements32;
+	const int idx = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+	if (idx >= elements32)
 	{
 		return;
 	}
-	const int row = static_cast<int>(idx / p.h);
-	const int column = static_cast<int>(idx % p.h);
+	const int row = static_cast<int>(static_cast<unsigned>(idx) / static_cast<unsigned>(p.h));
+	const int column = idx - row * p.h;
 	const int64_t slices = p.map.kept_count * chunk.count;
 	for (int64_t j = blockIdx.y; j < slices; j += gridDim.y)
 	{
