@@ -319,6 +319,25 @@ struct window_taps<REXLIB_HIP_KAISER_BESSEL>
 // The sample of stored element (row, column) of slice s: its value v * r *
 // (numerator weight) before the sign, its denominator weight, and false when
 // it is not inserted.
+// sin x for the CTF phase x (radians, |x| up to about 1e5). Cody-Waite
+// reduction: x = k pi/2 + r with the product k pi/2 taken in double, so that
+// r is exact to float precision; then the Taylor polynomials of sin (through
+// r^9) and cos (through r^8) on |r| <= pi/4, whose truncation errors are
+// below 2e-9 and 2.5e-8, selected and signed by the quadrant k mod 4.
+// (The library sinf takes its multi-word reduction for phases of a few
+// hundred radians, several times the cost of the whole sample.)
+__host__ __device__ inline float phase_sin(float x)
+{
+	const float k = rintf(x * 0.636619772367581343f); // 2 / pi
+	const float r = static_cast<float>(fma(-static_cast<double>(k), 1.5707963267948966192, static_cast<double>(x)));
+	const float r2 = r * r;
+	const float s = r + r * r2 * (-1.0f / 6.0f + r2 * (1.0f / 120.0f + r2 * (-1.0f / 5040.0f + r2 * (1.0f / 362880.0f))));
+	const float c = 1.0f + r2 * (-0.5f + r2 * (1.0f / 24.0f + r2 * (-1.0f / 720.0f + r2 * (1.0f / 40320.0f))));
+	const int q = static_cast<int>(k) & 3;
+	const float v = (q & 1) ? c : s;
+	return (q & 2) ? -v : v;
+}
+
 template <int VARIANT>
 __device__ inline bool prepare_sample(const insertion_params &p, const slice_entry &e, int64_t s, int row,
 	int column, float2 &value, float &count)
@@ -360,7 +379,7 @@ __device__ inline bool prepare_sample(const insertion_params &p, const slice_ent
 		const float norm = xx + yy;
 		const float chi = e.ctf_radial * norm + e.ctf_axial * (xx - yy) + e.ctf_diagonal * (fx * fy) -
 			e.ctf_aberration * (norm * norm) + e.ctf_phase;
-		numerator_weight = sinf(chi);
+		numerator_weight = phase_sin(chi);
 		denominator_weight = numerator_weight * numerator_weight;
 	}
 	const float scale = r * numerator_weight;
