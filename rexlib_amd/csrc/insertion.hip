@@ -584,6 +584,15 @@ __host__ __device__ constexpr int brick_pitch_z(int extent_y)
 	return pitch_y * extent_y + 1;
 }
 
+// A prepared sample of a slice: v r w (the value with its radial and
+// numerator weights) and r w_den, 12 bytes; zero when not inserted.
+struct prepared_sample
+{
+	float re;
+	float im;
+	float den;
+};
+
 // A voxel of a brick in LDS: 12 bytes.
 struct brick_voxel
 {
@@ -661,7 +670,7 @@ struct slice_chunk
 };
 
 template <int VARIANT>
-__global__ __launch_bounds__(256) void prepare_kernel(insertion_params p, slice_chunk chunk, float4 *prepared)
+__global__ __launch_bounds__(256) void prepare_kernel(insertion_params p, slice_chunk chunk, prepared_sample *prepared)
 {
 	// A slice holds n (n/2 + 1) < 2^31 elements: 32-bit indices (a 64-bit
 	// division per element would cost more than the sample itself).
@@ -687,7 +696,7 @@ __global__ __launch_bounds__(256) void prepare_kernel(insertion_params p, slice_
 			value = make_float2(0.0f, 0.0f);
 			weight = 0.0f;
 		}
-		prepared[j * elements + idx] = make_float4(value.x, value.y, weight, 0.0f);
+		prepared[j * elements + idx] = prepared_sample{value.x, value.y, weight};
 	}
 }
 
@@ -880,7 +889,7 @@ __device__ inline void bound_rows(const insertion_params &p, const float *x_step
 // the whole wavefront takes part.
 template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
 __device__ void insert_slice_sign(const insertion_params &p, const float *x_step, const float *y_step_f,
-	int last_shell, const float4 *slice, const brick_box &box, const float *lower, const float *upper, int sign_index,
+	int last_shell, const prepared_sample *slice, const brick_box &box, const float *lower, const float *upper, int sign_index,
 	int r0, int r1, brick_voxel *acc, int *row_table)
 {
 	const int lane = threadIdx.x & 63;
@@ -992,14 +1001,14 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 		// is checked against the slice before use.
 		if (class_total[0] <= 64 && class_total[1] <= 64 && class_total[2] <= 64 && class_total[3] <= 64)
 		{
-			float4 q[4];
+			prepared_sample q[4];
 			int sample_x[4];
 			int sample_y[4];
 			bool valid[4];
 #pragma unroll
 			for (int c = 0; c < 4; ++c)
 			{
-				q[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+				q[c] = prepared_sample{0.0f, 0.0f, 0.0f};
 				sample_x[c] = 0;
 				sample_y[c] = 0;
 				valid[c] = false;
@@ -1036,7 +1045,7 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 				}
 				// Lanes without a sample, and zero samples (not inserted), stay
 				// in step with the other lanes and add nothing.
-				const bool active = valid[c] && !(q[c].x == 0.0f && q[c].y == 0.0f && q[c].z == 0.0f);
+				const bool active = valid[c] && !(q[c].re == 0.0f && q[c].im == 0.0f && q[c].den == 0.0f);
 				const float fx = static_cast<float>(sample_x[c]);
 				const float fy = static_cast<float>(sample_y[c]);
 				float position[3];
@@ -1044,8 +1053,8 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 				{
 					position[a] = active ? fy * y_step[a] + fx * step[a] : -1.0e6f;
 				}
-				add_sample<PROFILE, ODD, ATOMIC, EY, EZ>(p, box, acc, position, q[c].x,
-					sign_index == 0 ? q[c].y : -q[c].y, q[c].z);
+				add_sample<PROFILE, ODD, ATOMIC, EY, EZ>(p, box, acc, position, q[c].re,
+					sign_index == 0 ? q[c].im : -q[c].im, q[c].den);
 			}
 			continue;
 		}
@@ -1054,13 +1063,13 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 		// afterwards one slot at a time (one class per instruction).
 		for (int group = 0; group < slots; group += max_slots)
 		{
-			float4 q[max_slots];
+			prepared_sample q[max_slots];
 			int sample_x[max_slots];
 			int sample_y[max_slots];
 #pragma unroll
 			for (int k = 0; k < max_slots; ++k)
 			{
-				q[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+				q[k] = prepared_sample{0.0f, 0.0f, 0.0f};
 				sample_x[k] = 0;
 				sample_y[k] = 0;
 				const int g = group + k;
@@ -1108,7 +1117,7 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 					continue;
 				}
 				// Zero samples (not inserted) stay in step with the other lanes.
-				const bool active = !(q[k].x == 0.0f && q[k].y == 0.0f && q[k].z == 0.0f);
+				const bool active = !(q[k].re == 0.0f && q[k].im == 0.0f && q[k].den == 0.0f);
 				const float fx = static_cast<float>(sample_x[k]);
 				const float fy = static_cast<float>(sample_y[k]);
 				float c[3];
@@ -1116,15 +1125,15 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 				{
 					c[a] = active ? fy * y_step[a] + fx * step[a] : -1.0e6f;
 				}
-				add_sample<PROFILE, ODD, ATOMIC, EY, EZ>(p, box, acc, c, q[k].x, sign_index == 0 ? q[k].y : -q[k].y,
-					q[k].z);
+				add_sample<PROFILE, ODD, ATOMIC, EY, EZ>(p, box, acc, c, q[k].re, sign_index == 0 ? q[k].im : -q[k].im,
+					q[k].den);
 			}
 		}
 	}
 }
 
 template <int PROFILE, bool ODD, int EY, int EZ>
-__global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chunk chunk, const float4 *prepared,
+__global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chunk chunk, const prepared_sample *prepared,
 	const int4 *tasks, int task_count, float reach)
 {
 	constexpr int pitch_z = brick_pitch_z(EY);
@@ -1221,7 +1230,7 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 				const int bit = __ffsll(static_cast<long long>(mask)) - 1;
 				mask &= mask - 1;
 				const int64_t js = window + bit;
-				const float4 *slice = prepared + (kept * chunk.count + js) * elements;
+				const prepared_sample *slice = prepared + (kept * chunk.count + js) * elements;
 				// The hit slice's frame, read from the lane that tested it.
 				const float x_step[3] = {lane_value(x0, bit), lane_value(x1, bit), lane_value(x2, bit)};
 				const float y_step[3] = {lane_value(y0, bit), lane_value(y1, bit), lane_value(y2, bit)};
@@ -1319,7 +1328,7 @@ int brick_ez_choice()
 }
 
 template <int PROFILE, int EY, int EZ>
-void launch_brick_shape(const insertion_params &p, slice_chunk chunk, const float4 *prepared, const int4 *tasks,
+void launch_brick_shape(const insertion_params &p, slice_chunk chunk, const prepared_sample *prepared, const int4 *tasks,
 	int task_count, float reach, hipStream_t stream)
 {
 	const dim3 grid(static_cast<unsigned>(task_count), static_cast<unsigned>(p.map.kept_count));
@@ -1334,7 +1343,7 @@ void launch_brick_shape(const insertion_params &p, slice_chunk chunk, const floa
 }
 
 template <int PROFILE>
-void launch_brick(const insertion_params &p, slice_chunk chunk, const float4 *prepared, const int4 *tasks,
+void launch_brick(const insertion_params &p, slice_chunk chunk, const prepared_sample *prepared, const int4 *tasks,
 	int task_count, float reach, hipStream_t stream)
 {
 	const int ey = brick_ey_choice();
@@ -1361,7 +1370,7 @@ void launch_brick(const insertion_params &p, slice_chunk chunk, const float4 *pr
 	}
 }
 
-void dispatch_brick(const insertion_params &p, slice_chunk chunk, const float4 *prepared, const int4 *tasks,
+void dispatch_brick(const insertion_params &p, slice_chunk chunk, const prepared_sample *prepared, const int4 *tasks,
 	int task_count, float reach, hipStream_t stream)
 {
 	switch (p.profile)
@@ -1378,7 +1387,7 @@ void dispatch_brick(const insertion_params &p, slice_chunk chunk, const float4 *
 	}
 }
 
-void dispatch_prepare(const insertion_params &p, slice_chunk chunk, float4 *prepared, hipStream_t stream)
+void dispatch_prepare(const insertion_params &p, slice_chunk chunk, prepared_sample *prepared, hipStream_t stream)
 {
 	const int64_t elements = static_cast<int64_t>(p.n) * p.h;
 	const dim3 grid(static_cast<unsigned>((elements + 255) / 256),
@@ -1582,13 +1591,13 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 	const bool brick = schedule != schedule_scatter && p.taps <= 4 && p.map.kept_count <= 65535;
 	if (brick)
 	{
-		// Prepared samples take 16 bytes per stored element; the slices of
+		// Prepared samples take 12 bytes per stored element; the slices of
 		// one call go through in chunks that keep them within 4 GiB.
-		const int64_t per_slice = static_cast<int64_t>(p.n) * p.h * static_cast<int64_t>(sizeof(float4)) *
+		const int64_t per_slice = static_cast<int64_t>(p.n) * p.h * static_cast<int64_t>(sizeof(prepared_sample)) *
 			p.map.kept_count;
 		const int64_t cap = int64_t(4) << 30;
 		const int64_t chunk_count = std::max<int64_t>(1, std::min<int64_t>(p.map.reduced_count, cap / per_slice));
-		float4 *prepared = static_cast<float4 *>(workspace_area_bytes(area_prepared,
+		prepared_sample *prepared = static_cast<prepared_sample *>(workspace_area_bytes(area_prepared,
 			static_cast<std::size_t>(per_slice * chunk_count), stream));
 		int count = 0;
 		const int4 *tasks = brick_tasks(p.big, chunk_count, count);
