@@ -459,6 +459,123 @@ __global__ void fft_lines_kernel(const float2 *src, line_set src_lines, float2 *
 	}
 }
 
+// 256-point lines by the four-step decomposition 256 = 16 x 16:
+//   X[k1 + 16 k2] = sum_n1 W16^(n1 k2) W256^(n1 k1) sum_n2 x[n1 + 16 n2] W16^(n2 k1).
+// Thread (line l, n1) holds x[n1 + 16 n2], n2 = 0..15, and computes the inner
+// 16-point DFT in registers (radix 4 x 4), multiplies by W256^(n1 k1), and
+// hands the values over through LDS to thread (l, k1), which computes the
+// outer 16-point DFT. One LDS exchange per line instead of four.
+constexpr int fft256_lines = 16; // lines per block of 256 threads
+
+// exp(sign 2 pi i m / 16) for m = 0..9 (the products n1 k1 of a 4 x 4 split).
+__device__ inline float2 unit16(int m, float sign)
+{
+	constexpr float c1 = 0.923879532511286756128f; // cos(pi / 8)
+	constexpr float s1 = 0.382683432365089771728f; // sin(pi / 8)
+	constexpr float c2 = 0.707106781186547524401f; // cos(pi / 4)
+	float c = 1.0f;
+	float s = 0.0f;
+	switch (m)
+	{
+	case 1: c = c1; s = s1; break;
+	case 2: c = c2; s = c2; break;
+	case 3: c = s1; s = c1; break;
+	case 4: c = 0.0f; s = 1.0f; break;
+	case 6: c = -c2; s = c2; break;
+	case 9: c = -c1; s = -s1; break;
+	default: break;
+	}
+	return make_float2(c, sign * s);
+}
+
+// In-register 16-point DFT: v[k] = sum_n v[n] exp(sign 2 pi i n k / 16).
+__device__ inline void dft16(float2 *v, float sign)
+{
+	float2 a[16];
+#pragma unroll
+	for (int n1 = 0; n1 < 4; ++n1)
+	{
+		float2 q[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+		dft<4>(q, sign);
+#pragma unroll
+		for (int k1 = 0; k1 < 4; ++k1)
+		{
+			a[4 * n1 + k1] = (n1 * k1 == 0) ? q[k1] : cmul(q[k1], unit16(n1 * k1, sign));
+		}
+	}
+#pragma unroll
+	for (int k1 = 0; k1 < 4; ++k1)
+	{
+		float2 q[4] = {a[k1], a[4 + k1], a[8 + k1], a[12 + k1]};
+		dft<4>(q, sign);
+#pragma unroll
+		for (int k2 = 0; k2 < 4; ++k2)
+		{
+			v[k1 + 4 * k2] = q[k2];
+		}
+	}
+}
+
+__global__ __launch_bounds__(256) void fft256_lines_kernel(const float2 *src, line_set src_lines, float2 *dst,
+	line_set dst_lines, const float2 *tw, float sign, float scale)
+{
+	// Per line 16 x 17 values (odd pitches against bank conflicts).
+	__shared__ float2 exchange[fft256_lines * (16 * 17) + 1];
+	__shared__ float2 twl[256];
+	__shared__ int64_t line_src[fft256_lines];
+	__shared__ int64_t line_dst[fft256_lines];
+	const int t = threadIdx.x;
+	const int l = t & (fft256_lines - 1);
+	const int j = t / fft256_lines; // n1 in the first half, k1 in the second
+	twl[t] = tw[t];
+	const int64_t total = src_lines.count;
+	for (int64_t first = blockIdx.x * static_cast<int64_t>(fft256_lines); first < total;
+		first += static_cast<int64_t>(gridDim.x) * fft256_lines)
+	{
+		const int lines = static_cast<int>(total - first < fft256_lines ? total - first : fft256_lines);
+		if (t < fft256_lines)
+		{
+			line_src[t] = t < lines ? line_offset(src_lines, first + t) : 0;
+			line_dst[t] = t < lines ? line_offset(dst_lines, first + t) : 0;
+		}
+		__syncthreads();
+		float2 v[16];
+		const bool live = l < lines;
+		const float2 *in = src + line_src[l];
+#pragma unroll
+		for (int k = 0; k < 16; ++k)
+		{
+			v[k] = live ? in[(j + 16 * k) * src_lines.s_elem] : make_float2(0.0f, 0.0f);
+		}
+		dft16(v, sign);
+		float2 *x = exchange + l * (16 * 17);
+#pragma unroll
+		for (int k1 = 0; k1 < 16; ++k1)
+		{
+			float2 w = twl[(j * k1) & 255];
+			w.y *= -sign; // the table holds exp(-2 pi i m / 256)
+			x[k1 * 17 + j] = k1 == 0 ? v[k1] : cmul(v[k1], w);
+		}
+		__syncthreads();
+#pragma unroll
+		for (int n1 = 0; n1 < 16; ++n1)
+		{
+			v[n1] = x[j * 17 + n1];
+		}
+		dft16(v, sign);
+		if (live)
+		{
+			float2 *out = dst + line_dst[l];
+#pragma unroll
+			for (int k2 = 0; k2 < 16; ++k2)
+			{
+				out[(j + 16 * k2) * dst_lines.s_elem] = make_float2(v[k2].x * scale, v[k2].y * scale);
+			}
+		}
+		__syncthreads();
+	}
+}
+
 // Half-spectrum rows, two at a time, to real rows, scaled.
 template <int N>
 __global__ void irfft_rows_kernel(const float2 *src, line_set src_rows, float *dst, line_set dst_rows,
@@ -611,6 +728,13 @@ void launch_fft_lines(const float2 *src, const line_set &src_lines, float2 *dst,
 	const int group = lines_per_group(static_cast<int>(n));
 	const std::size_t shared = sizeof(float2) * (2 * group * (n + 1) + n) + 2 * group * sizeof(int64_t);
 	const unsigned grid = grid_for(src_lines.count, group);
+	if (n == 256)
+	{
+		const unsigned blocks = grid_for(src_lines.count, fft256_lines);
+		fft256_lines_kernel<<<blocks, 256, 0, stream>>>(src, src_lines, dst, dst_lines, tw, sign, scale);
+		REXLIB_HIP_CHECK(hipGetLastError());
+		return;
+	}
 	with_fixed_length(n, [&](auto length)
 		{
 			constexpr int N = decltype(length)::value;
