@@ -637,14 +637,16 @@ __device__ inline int wave_inclusive_scan(int value)
 
 // Prefix maximum over the 64 lanes (lane i gets the maximum of lanes 0..i),
 // by DPP row shifts within rows of 16 and row broadcasts across them.
+// Lanes without a source read 0 (bound_ctrl), which leaves the result
+// unchanged wherever lane 0 holds a value of at least 0.
 __device__ __forceinline__ int wave_prefix_max(int v)
 {
-	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xf, 0xf, false)); // row_shr:1
-	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xf, 0xf, false)); // row_shr:2
-	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xf, 0xf, false)); // row_shr:4
-	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xf, 0xf, false)); // row_shr:8
-	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xa, 0xf, false)); // row_bcast:15
-	v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xc, 0xf, false)); // row_bcast:31
+	v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true)); // row_shr:1
+	v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true)); // row_shr:2
+	v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true)); // row_shr:4
+	v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true)); // row_shr:8
+	v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, true)); // row_bcast:15
+	v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, true)); // row_bcast:31
 	return v;
 }
 
@@ -717,7 +719,18 @@ __device__ __forceinline__ void add_sample_even(const insertion_params &p, const
 #pragma unroll
 	for (int a = 0; a < 3; ++a)
 	{
-		g[a] = window_taps<PROFILE>::place(p, c[a], w[a]);
+		if constexpr (PROFILE == REXLIB_HIP_TRIANGLE)
+		{
+			// floor and fraction, as window_taps<TRIANGLE>::place.
+			const float below = floorf(c[a]);
+			g[a] = static_cast<int>(below);
+			w[a][1] = c[a] - below;
+			w[a][0] = 1.0f - w[a][1];
+		}
+		else
+		{
+			g[a] = window_taps<PROFILE>::place(p, c[a], w[a]);
+		}
 		local[a] = g[a] - box.lo[a];
 #pragma unroll
 		for (int t = 0; t < taps; ++t)
@@ -1016,25 +1029,28 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 				{
 					continue;
 				}
-				row_table[lane] = -1;
+				// The first sample of a class always starts a row, so lane 0
+				// holds a row index of at least 0 and the prefix maximum
+				// gives every sample lane the row it falls in.
+				row_table[lane] = 0;
 				lds_order();
 				if (count[c] > 0)
 				{
 					atomicMax(&row_table[class_start[c] & 63], lane);
 				}
 				lds_order();
-				const int r = wave_prefix_max(row_table[lane]);
+				const int r = wave_prefix_max(row_table[lane]) & 63;
 				lds_order();
-				const int rr = r & 63;
-				const int row_begin = __shfl(class_begin[c], rr, 64);
-				const int row_start = __shfl(class_start[c], rr, 64);
-				const int sky = rc + rr;
+				const int row_begin = __shfl(class_begin[c], r, 64);
+				const int row_start = __shfl(class_start[c], r, 64);
+				const int sky = rc + r;
 				const int kx = row_begin + 2 * (lane - row_start);
 				const int row = sky < 0 ? sky + n : sky;
 				sample_x[c] = kx;
 				sample_y[c] = sky;
-				valid[c] = lane < class_total[c] && r >= 0 && row >= 0 && row < n && kx >= 0 && kx < h;
-				q[c] = slice[valid[c] ? row * h + kx : 0];
+				valid[c] = lane < class_total[c];
+				// Clamped into the slice whatever the lane holds.
+				q[c] = slice[valid[c] ? min(max(row * h + kx, 0), n * h - 1) : 0];
 			}
 #pragma unroll
 			for (int c = 0; c < 4; ++c)
