@@ -6,8 +6,11 @@ orientations, CTF defocus 1-3 um with up to 0.1 um astigmatism (300 kV,
 Cs 2.7 mm, Q 0.1, 1.5 A pixels), white noise at SNR 0.1. One step is the whole
 reconstruction: zero the accumulators, then for every launch of 4,096 images
 rfft2 + CTF-weighted trilinear insertion into 256 x 256 x 129 complex64 +
-float32 accumulators, then the weight quotient, irfft3 and gridding
-correction. Inputs are resident in HBM before the timed region.
+float32 accumulators (the fused image entry point, rexlib_hip_insert_ctf_images,
+on the float32 images), then the weight quotient, irfft3 and gridding
+correction. Inputs are resident in HBM before the timed region. The same
+reconstruction through the two calls of the rexlib API (rfft2, then
+insert_ctf_slices) is timed beside it.
 
 Printed: one JSON line with the throughput, the roofline of the dominant
 kernel, the CPU baseline (the reference's own insertion kernels, compiled from
@@ -205,31 +208,31 @@ def batched_form(torch, count=16, seed=2000):
 
 
 def parity_sample(args, stack, torch, em, rexlib_amd, count=256):
-    """rel-L2 of the GPU accumulators against the reference's float32 kernels on
-    the first `count` images."""
+    """rel-L2 of the GPU accumulators (fused rfft2 + insertion, as timed) on the
+    first `count` images against the reference's CPU path: pocketfft rfft2 in
+    float32 (scipy.fft) and the reference's float32 insertion kernels."""
     import oracle
+    import scipy.fft
     n, big = args.n, args.big
     h = n // 2 + 1
     images = stack.images[:count]
     num = torch.zeros((big, big, big // 2 + 1), dtype=torch.complex64, device="cuda")
     den = torch.zeros((big, big, big // 2 + 1), dtype=torch.float32, device="cuda")
-    spectra = rexlib_amd.rfft2(images)
-    em.insert_ctf_slices(num, den, spectra, torch.from_numpy(stack.transforms[:count].astype(np.float32)).cuda(),
+    em.insert_ctf_images(num, den, images, torch.from_numpy(stack.transforms[:count].astype(np.float32)).cuda(),
                          torch.from_numpy(stack.ctf[:count].astype(np.float32)).cuda(),
-                         torch.ones(h, device="cuda"), em.GriddingWindow.linear(), em.CtfOptics(300, 2.7, 0.1), 1.5,
-                         numerator_out=num, denominator_out=den)
+                         torch.ones(h, device="cuda"), em.GriddingWindow.linear(), em.CtfOptics(300, 2.7, 0.1), 1.5)
     torch.cuda.synchronize()
     acc = np.zeros((big, big, big // 2 + 1), np.complex64)
-    host_spectra = spectra.cpu().numpy()
+    threads = min(16, os.cpu_count() or 1)
+    host_spectra = scipy.fft.rfft2(images.cpu().numpy(), workers=threads).astype(np.complex64)
     inputs = [acc, acc.real.copy(), host_spectra, stack.transforms[:count].astype(np.float32),
               stack.ctf[:count].astype(np.float32), np.ones(h, np.float32)]
     if oracle.ref_available():
-        ref = oracle.ref_insert("ctf", inputs, oracle.Window.linear(), pixel_size=1.5,
-                                threads=min(16, os.cpu_count() or 1))
-        against = "reference kernels, float32"
+        ref = oracle.ref_insert("ctf", inputs, oracle.Window.linear(), pixel_size=1.5, threads=threads)
+        against = "pocketfft rfft2 (float32) + the reference's insertion kernels (float32)"
     else:
         ref = oracle.insert_f64("ctf", inputs, oracle.Window.linear(), pixel_size=1.5)
-        against = "double-precision oracle"
+        against = "pocketfft rfft2 (float32) + the double-precision oracle"
     return {
         "images": count,
         "against": against,
@@ -297,6 +300,16 @@ def main():
         num.zero_()
         den.zero_()
         for s, e in launches:
+            em.insert_ctf_images(num, den, stack.images[s:e], transforms[s:e], ctf[s:e], radial, window, optics, 1.5)
+        volume = rexlib_amd.irfft3(em.normalize(num, den, 1e-3))
+        return em.gridding_correct(volume, window, out=volume)
+
+    def api_step():
+        # The rexlib call sequence: rfft2 of each launch into a spectra
+        # buffer, then insert_ctf_slices accumulating in place.
+        num.zero_()
+        den.zero_()
+        for s, e in launches:
             out = spectra[: e - s]
             rexlib_amd.rfft2(stack.images[s:e], out=out)
             em.insert_ctf_slices(num, den, out, transforms[s:e], ctf[s:e], radial, window, optics, 1.5,
@@ -321,6 +334,10 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     value = args.images * args.steps / elapsed
     log(f"{args.steps} steps of {args.images} images: {elapsed:.3f} s, {value:,.0f} images/s")
+    api_ms = timed(torch, api_step)
+    two_call = {"images_per_s": args.images / (api_ms * 1e-3), "ms_per_step": api_ms,
+                "calls": "rfft2 + insert_ctf_slices per launch"}
+    log(f"two-call API: {two_call}")
 
     # Average over launches; the final partial launch holds images % L images.
     images_timed = args.images * args.steps
@@ -340,7 +357,7 @@ def main():
         "launches": brick_launches,
         "images_per_launch": images_per_launch,
         "bytes_per_image": b_img,
-        "prepare_kernel_ms": prepare_ms / prepare_launches if prepare_launches else None,
+        "column_prepare_kernel_ms": prepare_ms / prepare_launches if prepare_launches else None,
         "pipeline_frac": value * b_img / (HBM_PEAK_GBS * 1e9),
     }
 
@@ -378,14 +395,15 @@ def main():
         "dtype": "f32",
         "data": "synthetic (blob volume, exact projections, CTF, SNR 0.1)",
         "config": {
-            "workload": (f"plain FP32: {args.images} images {n}^2 -> {big}^3, launches of {L}, CTF + trilinear, "
-                         "then quotient, irfft3, gridding correction"),
+            "workload": (f"plain FP32: {args.images} images {n}^2 -> {big}^3, launches of {L}, fused rfft2 + "
+                         "CTF trilinear insertion, then quotient, irfft3, gridding correction"),
             "images": args.images, "n": n, "N": big, "launch": L, "window": "linear", "variant": "ctf",
             "brick": {"extent_x": 16, "extent_y": tuned[0], "extent_z": tuned[1], "share": tuned[2]} if tuned else None,
         },
         "roofline": roofline,
         "cpu_baseline": baseline,
         "parity": parity,
+        "two_call_api": two_call,
         "forms": forms,
     }
     print(json.dumps(line), flush=True)

@@ -114,7 +114,9 @@ void rexlib_hip_get_insertion_tuning(int32_t *extent_y, int32_t *extent_z, int32
 typedef enum rexlib_hip_kernel
 {
     REXLIB_HIP_KERNEL_BRICK = 0,   /* brick insertion kernel */
-    REXLIB_HIP_KERNEL_PREPARE = 1, /* per-sample preparation for the brick kernel */
+    REXLIB_HIP_KERNEL_PREPARE = 1, /* per-sample preparation for the brick kernel; for 256-point
+                                      images through rexlib_hip_insert_ctf_images, the column
+                                      transforms fused with it */
     REXLIB_HIP_KERNEL_SCATTER = 2  /* atomic scatter insertion kernel */
 } rexlib_hip_kernel;
 

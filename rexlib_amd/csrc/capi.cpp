@@ -236,6 +236,7 @@ struct insertion_request
 	double pixel_size;
 	int schedule;
 	hipStream_t stream;
+	const pending_columns *columns = nullptr;
 };
 
 // The shared body of the three insertions.
@@ -378,7 +379,7 @@ void insert(const insertion_request &r)
 	const rexlib_hip_array &den = *ops[slot_denominator];
 	const int64_t den_in_strides[3] = {core_stride(den, 3), core_stride(den, 2), core_stride(den, 1)};
 	run_insertion(p, r.schedule, static_cast<const float2 *>(num.data), num_in_strides,
-		static_cast<const float *>(den.data), den_in_strides, in_place, w.shape, r.stream);
+		static_cast<const float *>(den.data), den_in_strides, in_place, w.shape, r.columns, r.stream);
 }
 
 int insertion_schedule_from_env()
@@ -804,9 +805,9 @@ int rexlib_hip_insert_ctf_images(const rexlib_hip_array *numerator, const rexlib
 			const line_set out_rows{batch * n, 1, n, n * h, 0, h, 1};
 			launch_rfft_rows(kind, images->data, in_rows, static_cast<int>(m), scales, scale_stride, spectra, out_rows,
 				n, s);
-			const line_set columns{batch * h, 1, h, n * h, 0, 1, h};
-			launch_fft_lines(spectra, columns, spectra, columns, n, -1.0f, 1.0f, s);
 		}
+		// The column transforms are left to the insertion.
+		const pending_columns columns{spectra, line_set{batch * h, 1, h, n * h, 0, 1, h}};
 		rexlib_hip_array slices{};
 		slices.data = spectra;
 		slices.dtype = REXLIB_HIP_COMPLEX64;
@@ -819,7 +820,7 @@ int rexlib_hip_insert_ctf_images(const rexlib_hip_array *numerator, const rexlib
 		slices.strides[2] = 1;
 		insertion_request r{variant_ctf,
 			{numerator, denominator, &slices, transforms, radial_weights, ctf, numerator, denominator},
-			window, optics, pixel_size, insertion_schedule_from_env(), s};
+			window, optics, pixel_size, insertion_schedule_from_env(), s, &columns};
 		insert(r);
 	});
 }

@@ -19,6 +19,7 @@
 //     samples of every slice that reach it, and adds the brick to HBM once.
 //     This is the GPU form of the reference's tile-owned schedule
 //     (slice_insertion_functor.inl:105-156, 271-465).
+#include "fft_device.hpp"
 #include "launch.hpp"
 
 #include <algorithm>
@@ -338,9 +339,10 @@ __host__ __device__ inline float phase_sin(float x)
 	return (q & 2) ? -v : v;
 }
 
-template <int VARIANT>
-__device__ inline bool prepare_sample(const insertion_params &p, const slice_entry &e, int64_t s, int row,
-	int column, float2 &value, float &count)
+// The element's value comes from `load`, called only for inserted samples.
+template <int VARIANT, typename Load>
+__device__ inline bool prepare_value(const insertion_params &p, const slice_entry &e, int64_t s, int row,
+	int column, Load load, float2 &value, float &count)
 {
 	const int n = p.n;
 	if ((n % 2 == 0) && (row == n / 2 || column == n / 2))
@@ -359,7 +361,7 @@ __device__ inline bool prepare_sample(const insertion_params &p, const slice_ent
 	{
 		return false;
 	}
-	const float2 x = p.slices[e.slice_offset + row * p.slice_s0 + column * p.slice_s1];
+	const float2 x = load();
 	const float2 *table = p.tables + s * static_cast<int64_t>(p.h + n);
 	const float2 shifted = cmul(x, table[column]);
 	const float2 v = cmul(shifted, table[p.h + row]);
@@ -386,6 +388,14 @@ __device__ inline bool prepare_sample(const insertion_params &p, const slice_ent
 	value = make_float2(v.x * scale, v.y * scale);
 	count = r * denominator_weight;
 	return true;
+}
+
+template <int VARIANT>
+__device__ inline bool prepare_sample(const insertion_params &p, const slice_entry &e, int64_t s, int row,
+	int column, float2 &value, float &count)
+{
+	return prepare_value<VARIANT>(p, e, s, row, column,
+		[&] { return p.slices[e.slice_offset + row * p.slice_s0 + column * p.slice_s1]; }, value, count);
 }
 
 // Volume position of sample k_x of row k_y with the sign applied.
@@ -699,6 +709,133 @@ __global__ __launch_bounds__(256) void prepare_kernel(insertion_params p, slice_
 			weight = 0.0f;
 		}
 		prepared[j * elements + idx] = prepared_sample{value.x, value.y, weight};
+	}
+}
+
+// prepare_value without branches, for the fused column pass: every load is
+// in bounds whatever the sample (the radial index is clamped to the last
+// shell, the phase tables cover every row and column), and the sample is
+// zeroed afterwards where prepare_value rejects it. The loads of several rows
+// can then be in flight together. The arithmetic is prepare_value's, in the
+// same order.
+template <int VARIANT>
+__device__ inline prepared_sample prepare_unbranched(const insertion_params &p, const slice_entry &e,
+	const float2 *table, float2 x_phase, int row, int column, float2 x)
+{
+	const int n = p.n;
+	const int ky = row <= n / 2 ? row : row - n;
+	const int shell = shell_of(column, ky);
+	const bool nyquist = (n % 2 == 0) && (row == n / 2 || column == n / 2);
+	const float r = p.radial[e.radial_offset + min(shell, max(e.last_shell, 0)) * p.radial_s0];
+	const float2 y_phase = table[p.h + row];
+	float numerator_weight = 1.0f;
+	float denominator_weight = 1.0f;
+	if (VARIANT == variant_weighted)
+	{
+		denominator_weight = p.extra[e.extra_offset + row * p.extra_s0 + column * p.extra_s1];
+	}
+	else if (VARIANT == variant_ctf)
+	{
+		const float fx = static_cast<float>(column);
+		const float fy = static_cast<float>(ky);
+		const float xx = fx * fx;
+		const float yy = fy * fy;
+		const float norm = xx + yy;
+		const float chi = e.ctf_radial * norm + e.ctf_axial * (xx - yy) + e.ctf_diagonal * (fx * fy) -
+			e.ctf_aberration * (norm * norm) + e.ctf_phase;
+		numerator_weight = phase_sin(chi);
+		denominator_weight = numerator_weight * numerator_weight;
+	}
+	const float2 v = cmul(cmul(x, x_phase), y_phase);
+	const float scale = r * numerator_weight;
+	const bool inserted = !nyquist && shell <= e.last_shell && r != 0.0f;
+	return inserted ? prepared_sample{v.x * scale, v.y * scale, r * denominator_weight} :
+		prepared_sample{0.0f, 0.0f, 0.0f};
+}
+
+// The column transforms of 256-point slices fused with the preparation.
+// The rows of each slice are transformed already (in p.slices, at the
+// slices' own offsets and strides); a block transforms 16 columns k_x of the
+// chunk's slices by the four-step decomposition and writes the prepared
+// samples of those columns directly, so the finished spectra never pass
+// through HBM. The lines are (slice j of the chunk, k_x), numbered j h + k_x.
+template <int VARIANT>
+__global__ __launch_bounds__(256) void columns_prepare256_kernel(insertion_params p, slice_chunk chunk,
+	const float2 *tw, prepared_sample *prepared)
+{
+	__shared__ float2 exchange[fft256_lines * (16 * 17) + 1];
+	__shared__ float2 twl[256];
+	__shared__ int64_t line_slice[fft256_lines];
+	__shared__ int64_t line_src[fft256_lines];
+	__shared__ int64_t line_dst[fft256_lines];
+	__shared__ int line_column[fft256_lines];
+	const int t = threadIdx.x;
+	const int l = t & (fft256_lines - 1);
+	const int j = t / fft256_lines; // n1 in the first half of the transform, k1 in the second
+	twl[t] = tw[t];
+	const int h = p.h;
+	const int64_t elements = static_cast<int64_t>(p.n) * h;
+	const int64_t total = p.map.kept_count * chunk.count * h;
+	for (int64_t first = blockIdx.x * static_cast<int64_t>(fft256_lines); first < total;
+		first += static_cast<int64_t>(gridDim.x) * fft256_lines)
+	{
+		const int lines = static_cast<int>(total - first < fft256_lines ? total - first : fft256_lines);
+		if (t < fft256_lines)
+		{
+			int64_t slice = 0;
+			int64_t src = 0;
+			int64_t dst = 0;
+			int column = 0;
+			if (t < lines)
+			{
+				const int64_t line = first + t;
+				const int64_t jj = line / h;
+				column = static_cast<int>(line - jj * h);
+				const int64_t kept = jj / chunk.count;
+				slice = kept * p.map.reduced_count + chunk.begin + jj % chunk.count;
+				src = p.entries[slice].slice_offset + column * p.slice_s1;
+				dst = jj * elements + column;
+			}
+			line_slice[t] = slice;
+			line_src[t] = src;
+			line_dst[t] = dst;
+			line_column[t] = column;
+		}
+		__syncthreads();
+		const bool live = l < lines;
+		const float2 *in = p.slices + line_src[l];
+		float2 v[16];
+#pragma unroll
+		for (int k = 0; k < 16; ++k)
+		{
+			v[k] = live ? in[(j + 16 * k) * p.slice_s0] : make_float2(0.0f, 0.0f);
+		}
+		float2 *x = exchange + l * (16 * 17);
+		fft256_four_step(v, x, twl, j, -1.0f);
+		// The outputs go back to LDS (each thread to slots only it reads), so
+		// the preparation loops over them without holding all 16.
+		__syncthreads();
+#pragma unroll
+		for (int k2 = 0; k2 < 16; ++k2)
+		{
+			x[k2 * 17 + j] = v[k2];
+		}
+		if (live)
+		{
+			const int64_t slice = line_slice[l];
+			const slice_entry &e = p.entries[slice];
+			const int column = line_column[l];
+			const float2 *table = p.tables + slice * static_cast<int64_t>(h + p.n);
+			const float2 x_phase = table[column];
+			prepared_sample *out = prepared + line_dst[l];
+#pragma unroll 4
+			for (int k2 = 0; k2 < 16; ++k2)
+			{
+				const int row = j + 16 * k2;
+				out[row * h] = prepare_unbranched<VARIANT>(p, e, table, x_phase, row, column, x[k2 * 17 + j]);
+			}
+		}
+		__syncthreads();
 	}
 }
 
@@ -1422,6 +1559,26 @@ void dispatch_prepare(const insertion_params &p, slice_chunk chunk, prepared_sam
 	}
 }
 
+void dispatch_columns_prepare(const insertion_params &p, slice_chunk chunk, prepared_sample *prepared,
+	hipStream_t stream)
+{
+	const int64_t lines = p.map.kept_count * chunk.count * p.h;
+	const unsigned grid = static_cast<unsigned>(std::min<int64_t>((lines + fft256_lines - 1) / fft256_lines, 1 << 20));
+	const float2 *tw = twiddles(256);
+	switch (p.variant)
+	{
+	case variant_plain:
+		columns_prepare256_kernel<variant_plain><<<grid, 256, 0, stream>>>(p, chunk, tw, prepared);
+		break;
+	case variant_weighted:
+		columns_prepare256_kernel<variant_weighted><<<grid, 256, 0, stream>>>(p, chunk, tw, prepared);
+		break;
+	default:
+		columns_prepare256_kernel<variant_ctf><<<grid, 256, 0, stream>>>(p, chunk, tw, prepared);
+		break;
+	}
+}
+
 // Slices a task aims to hold: each brick is split into shares of about this
 // many expected plane hits.
 int brick_share_target()
@@ -1553,7 +1710,8 @@ void get_brick_tuning(int *extent_y, int *extent_z, int *share)
 
 // Host entry of every insertion: fills the workspace and launches.
 void run_insertion(insertion_params p, int schedule, const float2 *num_in, const int64_t *num_in_strides,
-	const float *den_in, const int64_t *den_in_strides, bool in_place, double shape, hipStream_t stream)
+	const float *den_in, const int64_t *den_in_strides, bool in_place, double shape, const pending_columns *columns,
+	hipStream_t stream)
 {
 	const int64_t slices = p.slice_count;
 	const int n = p.n;
@@ -1599,12 +1757,20 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 		volume_phase_kernel<<<(p.big + 255) / 256, 256, 0, stream>>>(vphase, p.big);
 		REXLIB_HIP_CHECK(hipGetLastError());
 	}
-	settle_kernel<<<static_cast<unsigned>(std::min<int64_t>(slices, 1 << 20)), 128, 0, stream>>>(p);
-	REXLIB_HIP_CHECK(hipGetLastError());
-
 	// The brick schedule needs the window to span at most a brick; wider
 	// Kaiser-Bessel windows take the scatter schedule.
 	const bool brick = schedule != schedule_scatter && p.taps <= 4 && p.map.kept_count <= 65535;
+	// Pending column transforms of 256-point slices run inside the
+	// preparation of each chunk; any others run here, in place.
+	const bool fused_columns = columns != nullptr && brick && p.n == 256;
+	if (columns != nullptr && !fused_columns && columns->lines.count > 0)
+	{
+		launch_fft_lines(columns->spectra, columns->lines, columns->spectra, columns->lines, p.n, -1.0f, 1.0f,
+			stream);
+	}
+	settle_kernel<<<static_cast<unsigned>(std::min<int64_t>(slices, 1 << 20)), 128, 0, stream>>>(p);
+	REXLIB_HIP_CHECK(hipGetLastError());
+
 	if (brick)
 	{
 		// Prepared samples take 12 bytes per stored element; the slices of
@@ -1623,7 +1789,14 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 		{
 			const slice_chunk chunk{begin, std::min(chunk_count, p.map.reduced_count - begin)};
 			const int prepare_token = timing_begin(REXLIB_HIP_KERNEL_PREPARE, stream);
-			dispatch_prepare(p, chunk, prepared, stream);
+			if (fused_columns)
+			{
+				dispatch_columns_prepare(p, chunk, prepared, stream);
+			}
+			else
+			{
+				dispatch_prepare(p, chunk, prepared, stream);
+			}
 			timing_end(prepare_token, stream);
 			const int brick_token = timing_begin(REXLIB_HIP_KERNEL_BRICK, stream);
 			dispatch_brick(p, chunk, prepared, tasks, count, reach, stream);

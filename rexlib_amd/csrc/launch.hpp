@@ -28,8 +28,20 @@ enum insertion_schedule : int
 	schedule_brick = 2
 };
 
+// Slices whose rows are transformed but whose columns are not yet (the
+// fused image insertion): the insertion runs the column transforms, for
+// 256-point slices together with the preparation of the samples.
+struct pending_columns
+{
+	float2 *spectra;
+	line_set lines;
+};
+
+// `columns`, when not null, describes the slices operand (transformed in
+// place where the transforms run on their own).
 void run_insertion(insertion_params p, int schedule, const float2 *num_in, const int64_t *num_in_strides,
-	const float *den_in, const int64_t *den_in_strides, bool in_place, double shape, hipStream_t stream);
+	const float *den_in, const int64_t *den_in_strides, bool in_place, double shape, const pending_columns *columns,
+	hipStream_t stream);
 
 void set_brick_tuning(int extent_y, int extent_z, int share);
 

@@ -67,6 +67,29 @@ def test_fused_images_equal_rfft2_then_insertion():
     assert oracle.rel_l2(den.cpu().numpy(), den2.cpu().numpy()) < 1e-6
 
 
+@pytest.mark.parametrize("schedule", ["auto", "scatter"])
+def test_fused_images_at_256_equal_rfft2_then_insertion(schedule, monkeypatch):
+    # 256-point slices take the column transforms fused with the sample
+    # preparation (brick schedule) or run them on their own (scatter).
+    monkeypatch.setenv("REXLIB_HIP_INSERTION_SCHEDULE", schedule)
+    stack = make_stack(count=40, m=128, n=256)
+    n, big = 256, 64
+    num, den = gpu_insert(stack, "f32", n, big)
+    images = torch.from_numpy(padded(stack.images.cpu().numpy(), n).astype(np.float32)).cuda()
+    slices = rexlib_amd.rfft2(images)
+    num2 = torch.zeros_like(num)
+    den2 = torch.zeros_like(den)
+    em.insert_ctf_slices(num2, den2, slices, to_device(stack.transforms), to_device(stack.ctf),
+                         torch.ones(n // 2 + 1, device="cuda"), em.GriddingWindow.linear(), OPTICS, 1.5,
+                         numerator_out=num2, denominator_out=den2)
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(num.cpu().numpy(), num2.cpu().numpy()) < 1e-6
+    assert oracle.rel_l2(den.cpu().numpy(), den2.cpu().numpy()) < 1e-6
+    ref = oracle_insert(stack, n, big)
+    assert oracle.rel_l2(num.cpu().numpy(), ref[0]) <= 1e-4
+    assert oracle.rel_l2(den.cpu().numpy(), ref[1]) <= 1e-4
+
+
 def test_float32_staging_with_padding_matches_the_oracle():
     stack = make_stack()
     n = big = 64
