@@ -666,6 +666,24 @@ __device__ inline float lane_value(float v, int lane)
 	return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// Lane i + 1's value by a DPP wavefront shift (wave_shl:1); lane 63 gets
+// `fallback`. Called with every lane of the wavefront active.
+__device__ __forceinline__ int next_lane(int v, int fallback)
+{
+	return __builtin_amdgcn_update_dpp(fallback, v, 0x130, 0xf, 0xf, false);
+}
+
+// Lane i - 1's value (wave_shr:1); lane 0 gets `fallback`.
+__device__ __forceinline__ int previous_lane(int v, int fallback)
+{
+	return __builtin_amdgcn_update_dpp(fallback, v, 0x138, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ float next_lane(float v)
+{
+	return __int_as_float(next_lane(__float_as_int(v), 0));
+}
+
 // Keeps the compiler from moving LDS accesses across it: the taps of other
 // lanes may alias, which it cannot see.
 __device__ inline void lds_order()
@@ -918,6 +936,118 @@ __device__ __forceinline__ void add_sample_even(const insertion_params &p, const
 	}
 }
 
+// add_sample_even for a slot that holds rows of one k_y parity with every
+// k_x, consecutive k_x of a row in consecutive lanes. Two samples of such a
+// slot fall in one voxel cell only if they are neighbours in a row (see
+// insert_slice_sign), so a lane whose cell is that of the lane before it
+// hands its sample to that lane, which adds both in one pass over the taps:
+// the read-modify-write of a tap then never has two lanes on one voxel.
+// Every lane of the wavefront calls it (inactive ones with active false).
+template <int PROFILE, int EY, int EZ>
+__device__ __forceinline__ void add_sample_pair(const insertion_params &p, const brick_box &box, brick_voxel *acc,
+	const float *c, bool active, float vr, float vi, float weight)
+{
+	static_assert(PROFILE != REXLIB_HIP_KAISER_BESSEL, "box and triangle windows only");
+	constexpr int taps = window_taps<PROFILE>::taps;
+	constexpr int pitch_z = brick_pitch_z(EY);
+	const int lane = threadIdx.x & 63;
+	float w[3][taps];
+	bool in[3][taps];
+	int g[3];
+	int local[3];
+#pragma unroll
+	for (int a = 0; a < 3; ++a)
+	{
+		if constexpr (PROFILE == REXLIB_HIP_TRIANGLE)
+		{
+			const float below = floorf(c[a]);
+			g[a] = static_cast<int>(below);
+			w[a][1] = c[a] - below;
+			w[a][0] = 1.0f - w[a][1];
+		}
+		else
+		{
+			g[a] = window_taps<PROFILE>::place(p, c[a], w[a]);
+		}
+		local[a] = g[a] - box.lo[a];
+#pragma unroll
+		for (int t = 0; t < taps; ++t)
+		{
+			in[a][t] = static_cast<unsigned>(local[a] + t) < static_cast<unsigned>(box.ext[a]);
+		}
+	}
+	// The cell of an active sample, which lies within a few voxels of the
+	// brick, in 10 bits per axis; inactive lanes hold keys no other lane has.
+	const int key = active ? (((local[0] + 512) & 1023) | (((local[1] + 512) & 1023) << 10) |
+		(((local[2] + 512) & 1023) << 20)) : ((1 << 30) | lane);
+	const bool merge = next_lane(key, -1) == key;
+	const bool follower = previous_lane(key, -1) == key;
+	// The next lane's sample (zero unless this lane takes it) and weights.
+	const float vr_next = next_lane(vr);
+	const float vi_next = next_lane(vi);
+	const float weight_next = next_lane(weight);
+	float w_next[3][taps];
+#pragma unroll
+	for (int a = 0; a < 3; ++a)
+	{
+		if constexpr (PROFILE == REXLIB_HIP_TRIANGLE)
+		{
+			w_next[a][1] = next_lane(w[a][1]);
+			w_next[a][0] = 1.0f - w_next[a][1];
+		}
+		else
+		{
+			w_next[a][0] = 1.0f;
+		}
+	}
+	const float nr = merge ? vr_next : 0.0f;
+	const float ni = merge ? vi_next : 0.0f;
+	const float nd = merge ? weight_next : 0.0f;
+	// A follower's sample is added by the lane before it.
+#pragma unroll
+	for (int t = 0; t < taps; ++t)
+	{
+		in[2][t] = in[2][t] && !follower;
+	}
+	const int base = local[0] + pitch_y * local[1] + pitch_z * local[2];
+	const bool odd = ((g[0] + g[1] + g[2]) & 1) != 0;
+	const float sr = odd ? -vr : vr;
+	const float si = odd ? -vi : vi;
+	const float tr = odd ? -nr : nr;
+	const float ti = odd ? -ni : ni;
+#pragma unroll
+	for (int tz = 0; tz < taps; ++tz)
+	{
+#pragma unroll
+		for (int ty = 0; ty < taps; ++ty)
+		{
+			const float wzy = w[2][tz] * w[1][ty];
+			const float wzy_next = w_next[2][tz] * w_next[1][ty];
+			const bool in_zy = in[2][tz] && in[1][ty];
+#pragma unroll
+			for (int tx = 0; tx < taps; ++tx)
+			{
+				const float ww = wzy * w[0][tx];
+				const float ww_next = wzy_next * w_next[0][tx];
+				const bool flip = ((tx + ty + tz) & 1) != 0;
+				const float re = (flip ? -sr : sr) * ww + (flip ? -tr : tr) * ww_next;
+				const float im = (flip ? -si : si) * ww + (flip ? -ti : ti) * ww_next;
+				const float de = weight * ww + nd * ww_next;
+				const int idx = base + tx + pitch_y * ty + pitch_z * tz;
+				if (in_zy && in[0][tx])
+				{
+					brick_voxel v = acc[idx];
+					v.re += re;
+					v.im += im;
+					v.den += de;
+					acc[idx] = v;
+				}
+				lds_order();
+			}
+		}
+	}
+}
+
 template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
 __device__ inline void add_sample(const insertion_params &p, const brick_box &box, brick_voxel *acc, const float *c,
 	float vr, float vi, float weight)
@@ -1143,6 +1273,74 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 		{
 			slots += (class_total[cls] + 63) >> 6;
 			slot_end[cls] = slots;
+		}
+		// Box and triangle windows on an even N, lattice-free slices: the two
+		// classes of one k_y parity share a slot whenever both fit. Samples of
+		// one slot then differ by a X + 2 b Y; unless (a, b) = (+-1, 0) that
+		// is at least sqrt(4 * 0.76) > sqrt(3) voxels (the bound behind
+		// lattice_free), more than a voxel cell spans, so only neighbours in a
+		// row can share a cell, and add_sample_pair merges those.
+		if constexpr (!ODD && PROFILE != REXLIB_HIP_KAISER_BESSEL && !ATOMIC)
+		{
+			const int pair_total[2] = {class_total[0] + class_total[1], class_total[2] + class_total[3]};
+			if (pair_total[0] <= 64 && pair_total[1] <= 64)
+			{
+				const int pair_start[2] = {class_start[0] + class_start[1], class_start[2] + class_start[3]};
+				const int pair_count[2] = {count[0] + count[1], count[2] + count[3]};
+				prepared_sample q[2];
+				int sample_x[2];
+				int sample_y[2];
+				bool valid[2];
+#pragma unroll
+				for (int c = 0; c < 2; ++c)
+				{
+					q[c] = prepared_sample{0.0f, 0.0f, 0.0f};
+					sample_x[c] = 0;
+					sample_y[c] = 0;
+					valid[c] = false;
+					if (pair_total[c] == 0)
+					{
+						continue;
+					}
+					row_table[lane] = 0;
+					lds_order();
+					if (pair_count[c] > 0)
+					{
+						atomicMax(&row_table[pair_start[c] & 63], lane);
+					}
+					lds_order();
+					const int r = wave_prefix_max(row_table[lane]) & 63;
+					lds_order();
+					const int row_begin = __shfl(begin, r, 64);
+					const int row_start = __shfl(pair_start[c], r, 64);
+					const int sky = rc + r;
+					const int kx = row_begin + (lane - row_start);
+					const int row = sky < 0 ? sky + n : sky;
+					sample_x[c] = kx;
+					sample_y[c] = sky;
+					valid[c] = lane < pair_total[c];
+					q[c] = slice[valid[c] ? min(max(row * h + kx, 0), n * h - 1) : 0];
+				}
+#pragma unroll
+				for (int c = 0; c < 2; ++c)
+				{
+					if (pair_total[c] == 0)
+					{
+						continue;
+					}
+					const bool active = valid[c] && !(q[c].re == 0.0f && q[c].im == 0.0f && q[c].den == 0.0f);
+					const float fx = static_cast<float>(sample_x[c]);
+					const float fy = static_cast<float>(sample_y[c]);
+					float position[3];
+					for (int a = 0; a < 3; ++a)
+					{
+						position[a] = active ? fy * y_step[a] + fx * step[a] : -1.0e6f;
+					}
+					add_sample_pair<PROFILE, EY, EZ>(p, box, acc, position, active, q[c].re,
+						sign_index == 0 ? q[c].im : -q[c].im, q[c].den);
+				}
+				continue;
+			}
 		}
 		// The common case, every class within one slot: class c is slot c,
 		// and the row of each sample comes from the wave's row table (each
