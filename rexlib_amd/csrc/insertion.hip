@@ -631,18 +631,45 @@ __device__ inline brick_box unpack_brick(int4 b)
 	return box;
 }
 
-__device__ inline int wave_inclusive_scan(int value)
+// Inclusive prefix sum over the 64 lanes by DPP row shifts (within rows of
+// 16) and row broadcasts (across them); lanes without a source add 0.
+__device__ __forceinline__ int wave_inclusive_scan(int v)
 {
-	const int lane = threadIdx.x & 63;
-	for (int offset = 1; offset < 64; offset <<= 1)
+	v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true); // row_shr:1
+	v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true); // row_shr:2
+	v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true); // row_shr:4
+	v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true); // row_shr:8
+	v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, true); // row_bcast:15
+	v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, true); // row_bcast:31
+	return v;
+}
+
+// Exclusive starts and wave totals of two per-lane counts (each at least 0).
+// Two 16-bit counts share an int while every count is below 1024 (64 lanes
+// then sum below 2^16), else each takes a scan of its own. Called with every
+// lane of the wavefront active.
+__device__ __forceinline__ void scan_two(int a, int b, int &start_a, int &start_b, int &total_a, int &total_b)
+{
+	if (__ballot(a > 1023 || b > 1023) == 0)
 	{
-		const int other = __shfl_up(value, offset, 64);
-		if (lane >= offset)
-		{
-			value += other;
-		}
+		const int v = a | (b << 16);
+		const int inclusive = wave_inclusive_scan(v);
+		const int exclusive = inclusive - v;
+		const int total = __builtin_amdgcn_readlane(inclusive, 63);
+		start_a = exclusive & 0xffff;
+		start_b = exclusive >> 16;
+		total_a = total & 0xffff;
+		total_b = total >> 16;
 	}
-	return value;
+	else
+	{
+		const int inclusive_a = wave_inclusive_scan(a);
+		const int inclusive_b = wave_inclusive_scan(b);
+		start_a = inclusive_a - a;
+		start_b = inclusive_b - b;
+		total_a = __builtin_amdgcn_readlane(inclusive_a, 63);
+		total_b = __builtin_amdgcn_readlane(inclusive_b, 63);
+	}
 }
 
 // Prefix maximum over the 64 lanes (lane i gets the maximum of lanes 0..i),
@@ -1231,49 +1258,6 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 				}
 			}
 		}
-		// Four parity classes; a class holds rows of one parity of k_y and the
-		// k_x of one parity within them. Each class is packed densely over the
-		// lanes (a scan over the rows), in slots of 64 samples.
-		int class_begin[4];
-		int class_start[4];
-		int class_total[4];
-		int slot_end[4]; // running count of slots through each class
-		int count[4];
-		for (int cls = 0; cls < 4; ++cls)
-		{
-			const int py = cls >> 1;
-			const int px = cls & 1;
-			class_begin[cls] = begin + ((begin - px) & 1); // first k_x >= begin of parity px
-			count[cls] = 0;
-			if (((ky - py) & 1) == 0 && class_begin[cls] < end)
-			{
-				count[cls] = (end - class_begin[cls] + 1) / 2;
-			}
-		}
-		// Two scans of two 16-bit counts each (a class holds fewer than
-		// 2^16 samples: at most 64 rows of at most a few hundred).
-		const int packed_a = count[0] | (count[1] << 16);
-		const int packed_b = count[2] | (count[3] << 16);
-		const int inclusive_a = wave_inclusive_scan(packed_a);
-		const int inclusive_b = wave_inclusive_scan(packed_b);
-		const int exclusive_a = inclusive_a - packed_a;
-		const int exclusive_b = inclusive_b - packed_b;
-		const int total_a = __shfl(inclusive_a, 63, 64);
-		const int total_b = __shfl(inclusive_b, 63, 64);
-		class_start[0] = exclusive_a & 0xffff;
-		class_start[1] = exclusive_a >> 16;
-		class_start[2] = exclusive_b & 0xffff;
-		class_start[3] = exclusive_b >> 16;
-		class_total[0] = total_a & 0xffff;
-		class_total[1] = total_a >> 16;
-		class_total[2] = total_b & 0xffff;
-		class_total[3] = total_b >> 16;
-		int slots = 0;
-		for (int cls = 0; cls < 4; ++cls)
-		{
-			slots += (class_total[cls] + 63) >> 6;
-			slot_end[cls] = slots;
-		}
 		// Box and triangle windows on an even N, lattice-free slices: the two
 		// classes of one k_y parity share a slot whenever both fit. Samples of
 		// one slot then differ by a X + 2 b Y; unless (a, b) = (+-1, 0) that
@@ -1282,11 +1266,14 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 		// row can share a cell, and add_sample_pair merges those.
 		if constexpr (!ODD && PROFILE != REXLIB_HIP_KAISER_BESSEL && !ATOMIC)
 		{
-			const int pair_total[2] = {class_total[0] + class_total[1], class_total[2] + class_total[3]};
+			// Per row: every k_x of [begin, end), counted for its k_y parity.
+			const int row_count = max(end - begin, 0);
+			const int pair_count[2] = {(ky & 1) == 0 ? row_count : 0, (ky & 1) != 0 ? row_count : 0};
+			int pair_start[2];
+			int pair_total[2];
+			scan_two(pair_count[0], pair_count[1], pair_start[0], pair_start[1], pair_total[0], pair_total[1]);
 			if (pair_total[0] <= 64 && pair_total[1] <= 64)
 			{
-				const int pair_start[2] = {class_start[0] + class_start[1], class_start[2] + class_start[3]};
-				const int pair_count[2] = {count[0] + count[1], count[2] + count[3]};
 				prepared_sample q[2];
 				int sample_x[2];
 				int sample_y[2];
@@ -1341,6 +1328,33 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 				}
 				continue;
 			}
+		}
+		// Four parity classes; a class holds rows of one parity of k_y and the
+		// k_x of one parity within them. Each class is packed densely over the
+		// lanes (a scan over the rows), in slots of 64 samples.
+		int class_begin[4];
+		int class_start[4];
+		int class_total[4];
+		int slot_end[4]; // running count of slots through each class
+		int count[4];
+		for (int cls = 0; cls < 4; ++cls)
+		{
+			const int py = cls >> 1;
+			const int px = cls & 1;
+			class_begin[cls] = begin + ((begin - px) & 1); // first k_x >= begin of parity px
+			count[cls] = 0;
+			if (((ky - py) & 1) == 0 && class_begin[cls] < end)
+			{
+				count[cls] = (end - class_begin[cls] + 1) / 2;
+			}
+		}
+		scan_two(count[0], count[1], class_start[0], class_start[1], class_total[0], class_total[1]);
+		scan_two(count[2], count[3], class_start[2], class_start[3], class_total[2], class_total[3]);
+		int slots = 0;
+		for (int cls = 0; cls < 4; ++cls)
+		{
+			slots += (class_total[cls] + 63) >> 6;
+			slot_end[cls] = slots;
 		}
 		// The common case, every class within one slot: class c is slot c,
 		// and the row of each sample comes from the wave's row table (each

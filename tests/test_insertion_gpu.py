@@ -210,6 +210,24 @@ def test_production_like_sizes_match_the_reference_kernels(variant, schedule):
         assert oracle.rel_l2(den, ref[1]) <= 1e-5
 
 
+def test_rows_longer_than_1023_samples(schedule):
+    # n = 4096 into N = 8: every row of the slice reaches the one brick with
+    # up to 2,049 samples, so the per-row counts no longer fit the 16-bit
+    # halves of a shared scan and each class takes a scan of its own.
+    # One sample in a thousand is nonzero (values and weights), which keeps
+    # the float32 sums of the 320 voxels short.
+    print(f"long rows n=4096 N=8 {schedule}", flush=True)
+    inputs = make_case("weighted", 4096, 8, 41, slices=1, zero_accumulators=True)
+    keep = np.random.default_rng(42).random(inputs[2].shape) < 1e-3
+    inputs[2] = inputs[2] * keep
+    inputs[3] = inputs[3] * keep
+    w = oracle.Window.linear()
+    num, den = hip_insert("weighted", inputs, w)
+    ref = oracle.insert_f64("weighted", inputs, w)
+    assert oracle.rel_l2(num, ref[0]) <= 1e-4
+    assert oracle.rel_l2(den, ref[1]) <= 1e-4
+
+
 def test_kernel_timing_records_each_launch(schedule):
     """Event timing brackets every launch of the schedule's insertion kernel
     and of nothing else; switching it off stops the records."""
