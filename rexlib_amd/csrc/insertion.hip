@@ -693,22 +693,22 @@ __device__ inline float lane_value(float v, int lane)
 	return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// Lane i + 1's value by a DPP wavefront shift (wave_shl:1); lane 63 gets
-// `fallback`. Called with every lane of the wavefront active.
-__device__ __forceinline__ int next_lane(int v, int fallback)
+// Lane i + 1's value by a DPP wavefront shift (wave_shl:1); lane 63 gets 0
+// (bound_ctrl). Called with every lane of the wavefront active.
+__device__ __forceinline__ int next_lane(int v)
 {
-	return __builtin_amdgcn_update_dpp(fallback, v, 0x130, 0xf, 0xf, false);
+	return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true);
 }
 
-// Lane i - 1's value (wave_shr:1); lane 0 gets `fallback`.
-__device__ __forceinline__ int previous_lane(int v, int fallback)
+// Lane i - 1's value (wave_shr:1); lane 0 gets 0.
+__device__ __forceinline__ int previous_lane(int v)
 {
-	return __builtin_amdgcn_update_dpp(fallback, v, 0x138, 0xf, 0xf, false);
+	return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true);
 }
 
 __device__ __forceinline__ float next_lane(float v)
 {
-	return __int_as_float(next_lane(__float_as_int(v), 0));
+	return __int_as_float(next_lane(__float_as_int(v)));
 }
 
 // Keeps the compiler from moving LDS accesses across it: the taps of other
@@ -1004,11 +1004,12 @@ __device__ __forceinline__ void add_sample_pair(const insertion_params &p, const
 		}
 	}
 	// The cell of an active sample, which lies within a few voxels of the
-	// brick, in 10 bits per axis; inactive lanes hold keys no other lane has.
+	// brick, in 10 bits per axis (never 0); inactive lanes hold keys no other
+	// lane has. The lanes at the ends read 0 for their missing neighbour.
 	const int key = active ? (((local[0] + 512) & 1023) | (((local[1] + 512) & 1023) << 10) |
 		(((local[2] + 512) & 1023) << 20)) : ((1 << 30) | lane);
-	const bool merge = next_lane(key, -1) == key;
-	const bool follower = previous_lane(key, -1) == key;
+	const bool merge = next_lane(key) == key;
+	const bool follower = previous_lane(key) == key;
 	// The next lane's sample (zero unless this lane takes it) and weights.
 	const float vr_next = next_lane(vr);
 	const float vi_next = next_lane(vi);
@@ -1057,16 +1058,15 @@ __device__ __forceinline__ void add_sample_pair(const insertion_params &p, const
 				const float ww = wzy * w[0][tx];
 				const float ww_next = wzy_next * w_next[0][tx];
 				const bool flip = ((tx + ty + tz) & 1) != 0;
-				const float re = (flip ? -sr : sr) * ww + (flip ? -tr : tr) * ww_next;
-				const float im = (flip ? -si : si) * ww + (flip ? -ti : ti) * ww_next;
-				const float de = weight * ww + nd * ww_next;
 				const int idx = base + tx + pitch_y * ty + pitch_z * tz;
 				if (in_zy && in[0][tx])
 				{
+					// Two fused multiply-adds per component: the partner's term,
+					// then this lane's.
 					brick_voxel v = acc[idx];
-					v.re += re;
-					v.im += im;
-					v.den += de;
+					v.re = fmaf(flip ? -sr : sr, ww, fmaf(flip ? -tr : tr, ww_next, v.re));
+					v.im = fmaf(flip ? -si : si, ww, fmaf(flip ? -ti : ti, ww_next, v.im));
+					v.den = fmaf(weight, ww, fmaf(nd, ww_next, v.den));
 					acc[idx] = v;
 				}
 				lds_order();
