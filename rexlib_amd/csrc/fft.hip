@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256) void fft256_lines_kernel(const float2 *src, li
 	const int j = t / fft256_lines; // n1 in the first half, k1 in the second
 	twl[t] = tw[t];
 	const int64_t total = src_lines.count;
-	for (int64_t first = blockIdx.x * static_cast<int64_t>(fft256_lines); first < total;
+	for (int64_t first = xcd_block(blockIdx.x, gridDim.x) * static_cast<int64_t>(fft256_lines); first < total;
 		first += static_cast<int64_t>(gridDim.x) * fft256_lines)
 	{
 		const int lines = static_cast<int>(total - first < fft256_lines ? total - first : fft256_lines);
@@ -593,7 +593,7 @@ void launch_fft_lines(const float2 *src, const line_set &src_lines, float2 *dst,
 	const unsigned grid = grid_for(src_lines.count, group);
 	if (n == 256)
 	{
-		const unsigned blocks = grid_for(src_lines.count, fft256_lines);
+		const unsigned blocks = xcd_grid(grid_for(src_lines.count, fft256_lines));
 		fft256_lines_kernel<<<blocks, 256, 0, stream>>>(src, src_lines, dst, dst_lines, tw, sign, scale);
 		REXLIB_HIP_CHECK(hipGetLastError());
 		return;

@@ -154,4 +154,22 @@ __device__ inline void fft256_four_step(float2 *v, float2 *line, const float2 *t
 	dft16(v, sign);
 }
 
+// The chip's eight XCDs take workgroups in turn (workgroup b runs on XCD
+// b mod 8), each behind its own L2. With a grid that is a multiple of 8,
+// workgroup b takes logical block (b mod 8) (G / 8) + b / 8: each XCD then
+// works through one contiguous range of logical blocks, so neighbouring
+// blocks, whose lines share cache lines, meet in one L2. Other grids keep
+// b.
+constexpr unsigned xcd_count = 8;
+
+__device__ inline unsigned xcd_block(unsigned b, unsigned grid)
+{
+	return grid % xcd_count == 0 ? (b % xcd_count) * (grid / xcd_count) + b / xcd_count : b;
+}
+
+__host__ inline unsigned xcd_grid(unsigned blocks)
+{
+	return (blocks + xcd_count - 1) / xcd_count * xcd_count;
+}
+
 } // namespace rexlib_hip

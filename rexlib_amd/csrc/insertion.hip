@@ -821,7 +821,7 @@ __global__ __launch_bounds__(256) void columns_prepare256_kernel(insertion_param
 	const int h = p.h;
 	const int64_t elements = static_cast<int64_t>(p.n) * h;
 	const int64_t total = p.map.kept_count * chunk.count * h;
-	for (int64_t first = blockIdx.x * static_cast<int64_t>(fft256_lines); first < total;
+	for (int64_t first = xcd_block(blockIdx.x, gridDim.x) * static_cast<int64_t>(fft256_lines); first < total;
 		first += static_cast<int64_t>(gridDim.x) * fft256_lines)
 	{
 		const int lines = static_cast<int>(total - first < fft256_lines ? total - first : fft256_lines);
@@ -1775,7 +1775,8 @@ void dispatch_columns_prepare(const insertion_params &p, slice_chunk chunk, prep
 	hipStream_t stream)
 {
 	const int64_t lines = p.map.kept_count * chunk.count * p.h;
-	const unsigned grid = static_cast<unsigned>(std::min<int64_t>((lines + fft256_lines - 1) / fft256_lines, 1 << 20));
+	const unsigned grid = xcd_grid(static_cast<unsigned>(
+		std::min<int64_t>((lines + fft256_lines - 1) / fft256_lines, 1 << 20)));
 	const float2 *tw = twiddles(256);
 	switch (p.variant)
 	{
