@@ -797,17 +797,20 @@ int rexlib_hip_insert_ctf_images(const rexlib_hip_array *numerator, const rexlib
 		}
 		const hipStream_t s = as_stream(stream);
 		const int64_t h = n / 2 + 1;
-		float2 *spectra = static_cast<float2 *>(
-			workspace_area_bytes(area_spectra, sizeof(float2) * static_cast<std::size_t>(std::max<int64_t>(batch, 1) * n * h), s));
+		// Rows of the half spectra start every 16 elements (128 bytes), so
+		// the column passes read whole cache lines.
+		const int64_t pitch = (h + 15) / 16 * 16;
+		float2 *spectra = static_cast<float2 *>(workspace_area_bytes(area_spectra,
+			sizeof(float2) * static_cast<std::size_t>(std::max<int64_t>(batch, 1) * n * pitch), s));
 		if (batch > 0)
 		{
 			const line_set in_rows{batch * m, 1, m, in_stride, 0, core_stride(*images, 2), core_stride(*images, 1)};
-			const line_set out_rows{batch * n, 1, n, n * h, 0, h, 1};
+			const line_set out_rows{batch * n, 1, n, n * pitch, 0, pitch, 1};
 			launch_rfft_rows(kind, images->data, in_rows, static_cast<int>(m), scales, scale_stride, spectra, out_rows,
 				n, s);
 		}
 		// The column transforms are left to the insertion.
-		const pending_columns columns{spectra, line_set{batch * h, 1, h, n * h, 0, 1, h}};
+		const pending_columns columns{spectra, line_set{batch * h, 1, h, n * pitch, 0, 1, pitch}};
 		rexlib_hip_array slices{};
 		slices.data = spectra;
 		slices.dtype = REXLIB_HIP_COMPLEX64;
@@ -815,8 +818,8 @@ int rexlib_hip_insert_ctf_images(const rexlib_hip_array *numerator, const rexlib
 		slices.extents[0] = batch;
 		slices.extents[1] = n;
 		slices.extents[2] = h;
-		slices.strides[0] = n * h;
-		slices.strides[1] = h;
+		slices.strides[0] = n * pitch;
+		slices.strides[1] = pitch;
 		slices.strides[2] = 1;
 		insertion_request r{variant_ctf,
 			{numerator, denominator, &slices, transforms, radial_weights, ctf, numerator, denominator},

@@ -803,7 +803,10 @@ __device__ inline prepared_sample prepare_unbranched(const insertion_params &p, 
 // slices' own offsets and strides); a block transforms 16 columns k_x of the
 // chunk's slices by the four-step decomposition and writes the prepared
 // samples of those columns directly, so the finished spectra never pass
-// through HBM. The lines are (slice j of the chunk, k_x), numbered j h + k_x.
+// through HBM. The Nyquist column k_x = n/2 holds no inserted sample and the
+// brick schedule never reads it, so it is skipped: the lines are (slice j of
+// the chunk, k_x < n/2), numbered j n/2 + k_x, and a block's 16 columns never
+// straddle two slices.
 template <int VARIANT>
 __global__ __launch_bounds__(256) void columns_prepare256_kernel(insertion_params p, slice_chunk chunk,
 	const float2 *tw, prepared_sample *prepared)
@@ -819,8 +822,9 @@ __global__ __launch_bounds__(256) void columns_prepare256_kernel(insertion_param
 	const int j = t / fft256_lines; // n1 in the first half of the transform, k1 in the second
 	twl[t] = tw[t];
 	const int h = p.h;
+	const int columns = p.n / 2;
 	const int64_t elements = static_cast<int64_t>(p.n) * h;
-	const int64_t total = p.map.kept_count * chunk.count * h;
+	const int64_t total = p.map.kept_count * chunk.count * columns;
 	for (int64_t first = xcd_block(blockIdx.x, gridDim.x) * static_cast<int64_t>(fft256_lines); first < total;
 		first += static_cast<int64_t>(gridDim.x) * fft256_lines)
 	{
@@ -834,8 +838,8 @@ __global__ __launch_bounds__(256) void columns_prepare256_kernel(insertion_param
 			if (t < lines)
 			{
 				const int64_t line = first + t;
-				const int64_t jj = line / h;
-				column = static_cast<int>(line - jj * h);
+				const int64_t jj = line / columns;
+				column = static_cast<int>(line - jj * columns);
 				const int64_t kept = jj / chunk.count;
 				slice = kept * p.map.reduced_count + chunk.begin + jj % chunk.count;
 				src = p.entries[slice].slice_offset + column * p.slice_s1;
@@ -1774,7 +1778,7 @@ void dispatch_prepare(const insertion_params &p, slice_chunk chunk, prepared_sam
 void dispatch_columns_prepare(const insertion_params &p, slice_chunk chunk, prepared_sample *prepared,
 	hipStream_t stream)
 {
-	const int64_t lines = p.map.kept_count * chunk.count * p.h;
+	const int64_t lines = p.map.kept_count * chunk.count * (p.n / 2);
 	const unsigned grid = xcd_grid(static_cast<unsigned>(
 		std::min<int64_t>((lines + fft256_lines - 1) / fft256_lines, 1 << 20)));
 	const float2 *tw = twiddles(256);
