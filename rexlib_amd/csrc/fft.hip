@@ -398,7 +398,7 @@ __global__ __launch_bounds__(256) void fft256_lines_kernel(const float2 *src, li
 	line_set dst_lines, const float2 *tw, float sign, float scale)
 {
 	// Per line 16 x 17 values (odd pitches against bank conflicts).
-	__shared__ float2 exchange[fft256_lines * (16 * 17) + 1];
+	__shared__ float2 exchange[fft256_lines * fft256_line_stride];
 	__shared__ float2 twl[256];
 	__shared__ int64_t line_src[fft256_lines];
 	__shared__ int64_t line_dst[fft256_lines];
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(256) void fft256_lines_kernel(const float2 *src, li
 		{
 			v[k] = live ? in[(j + 16 * k) * src_lines.s_elem] : make_float2(0.0f, 0.0f);
 		}
-		fft256_four_step(v, exchange + l * (16 * 17), twl, j, sign);
+		fft256_four_step(v, exchange + l * fft256_line_stride, twl, j, sign);
 		if (live)
 		{
 			float2 *out = dst + line_dst[l];

@@ -80,6 +80,10 @@ __device__ inline void dft<5>(float2 *v, float sign)
 // hands the values over through LDS to thread (l, k1), which computes the
 // outer 16-point DFT. One LDS exchange per line instead of four.
 constexpr int fft256_lines = 16; // lines per block of 256 threads
+// LDS float2 values per line of the exchange: 16 x 17 values (odd pitch within a
+// line) plus one, so that lines start 546 dwords apart, 2 banks on from one
+// another, and the 16 lines of a wave's accesses spread over all 32 banks.
+constexpr int fft256_line_stride = 16 * 17 + 1;
 
 // exp(sign 2 pi i m / 16) for m = 0..9 (the products n1 k1 of a 4 x 4 split).
 __device__ inline float2 unit16(int m, float sign)

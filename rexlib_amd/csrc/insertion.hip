@@ -811,7 +811,7 @@ template <int VARIANT>
 __global__ __launch_bounds__(256) void columns_prepare256_kernel(insertion_params p, slice_chunk chunk,
 	const float2 *tw, prepared_sample *prepared)
 {
-	__shared__ float2 exchange[fft256_lines * (16 * 17) + 1];
+	__shared__ float2 exchange[fft256_lines * fft256_line_stride];
 	__shared__ float2 twl[256];
 	__shared__ int64_t line_slice[fft256_lines];
 	__shared__ int64_t line_src[fft256_lines];
@@ -859,7 +859,7 @@ __global__ __launch_bounds__(256) void columns_prepare256_kernel(insertion_param
 		{
 			v[k] = live ? in[(j + 16 * k) * p.slice_s0] : make_float2(0.0f, 0.0f);
 		}
-		float2 *x = exchange + l * (16 * 17);
+		float2 *x = exchange + l * fft256_line_stride;
 		fft256_four_step(v, x, twl, j, -1.0f);
 		// The outputs go back to LDS (each thread to slots only it reads), so
 		// the preparation loops over them without holding all 16.
