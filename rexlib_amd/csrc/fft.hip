@@ -19,6 +19,7 @@
 #include <hip/hip_fp16.h>
 
 #include <cmath>
+#include <cstdint>
 #include <map>
 #include <mutex>
 #include <type_traits>
@@ -244,6 +245,37 @@ __device__ inline float load_real<real_f16>(const void *base, int64_t index)
 	return __half2float(static_cast<const __half *>(base)[index]);
 }
 
+// Four consecutive elements from `index` (a multiple of 4, with the base
+// aligned to four elements), as floats.
+template <int KIND>
+__device__ inline float4 load_real4(const void *base, int64_t index)
+{
+	if constexpr (KIND == real_f32)
+	{
+		return *reinterpret_cast<const float4 *>(static_cast<const float *>(base) + index);
+	}
+	else
+	{
+		const uint2 bits = *reinterpret_cast<const uint2 *>(static_cast<const unsigned short *>(base) + index);
+		const unsigned short e[4] = {static_cast<unsigned short>(bits.x & 0xffff), static_cast<unsigned short>(bits.x >> 16),
+			static_cast<unsigned short>(bits.y & 0xffff), static_cast<unsigned short>(bits.y >> 16)};
+		float f[4];
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+		{
+			if constexpr (KIND == real_bf16)
+			{
+				f[i] = __uint_as_float(static_cast<unsigned>(e[i]) << 16);
+			}
+			else
+			{
+				f[i] = __half2float(__ushort_as_half(e[i]));
+			}
+		}
+		return make_float4(f[0], f[1], f[2], f[3]);
+	}
+}
+
 // Rows of real images, two at a time, to half spectra. Images of extent m
 // are zero-padded to the transform length n with their origin m/2 moved to
 // n/2. `rows` counts the padded rows, n per image, in line set `in` of images
@@ -439,6 +471,163 @@ __global__ __launch_bounds__(256) void fft256_lines_kernel(const float2 *src, li
 	}
 }
 
+// Real 256-point rows, two to a complex line, by the four-step
+// decomposition: a block takes 16 lines (32 rows). Thread (line l, j) loads
+// elements j + 16 k of its two rows (16 consecutive elements per line and
+// load across the lanes), transforms them with fft256_four_step, and stores
+// Z[j + 16 k2] to LDS. The separation of the two half spectra
+// (X_a = (Z_k + conj Z_{n-k}) / 2, X_b = (Z_k - conj Z_{n-k}) / 2i) then
+// reads pairs from LDS, and writes each output row's 129 values, 16
+// consecutive per lane group. Rows of the padding, and columns outside the
+// image, are zeros, as in rfft_rows_kernel.
+template <int KIND, bool VEC>
+__global__ __launch_bounds__(256) void rfft256_rows_kernel(const void *in, line_set in_rows, int m,
+	const float *scales, int64_t scale_stride, float2 *out, line_set out_rows, const float2 *tw)
+{
+	constexpr int n = 256;
+	constexpr int h = n / 2 + 1;
+	__shared__ float2 exchange[fft256_lines * fft256_line_stride];
+	__shared__ float2 twl[256];
+	__shared__ int64_t row_in[2 * fft256_lines];
+	__shared__ int64_t row_out[2 * fft256_lines];
+	__shared__ float row_scale[2 * fft256_lines];
+	const int t = threadIdx.x;
+	const int j = t & 15;
+	const int l = t >> 4;
+	twl[t] = tw[t];
+	const int64_t total = out_rows.count; // padded rows of all images
+	const int offset = n / 2 - m / 2;
+	for (int64_t first = xcd_block(blockIdx.x, gridDim.x) * static_cast<int64_t>(2 * fft256_lines); first < total;
+		first += static_cast<int64_t>(gridDim.x) * 2 * fft256_lines)
+	{
+		if (t < 2 * fft256_lines)
+		{
+			const int64_t row = first + t;
+			int64_t src = -1;
+			int64_t dst = -1;
+			float scale = 1.0f;
+			if (row < total)
+			{
+				const int64_t image = row / n;
+				const int r = static_cast<int>(row - image * n) - offset;
+				if (r >= 0 && r < m)
+				{
+					src = line_offset(in_rows, image * m + r);
+					if (scales != nullptr)
+					{
+						scale = scales[image * scale_stride];
+					}
+				}
+				dst = line_offset(out_rows, row);
+			}
+			row_in[t] = src;
+			row_out[t] = dst;
+			row_scale[t] = scale;
+		}
+		__syncthreads();
+		float2 *x = exchange + l * fft256_line_stride;
+		float2 v[16];
+		if constexpr (VEC)
+		{
+			// Rows whose every run of 4 elements is aligned to 4 elements:
+			// each thread loads 4 consecutive elements of a row pair at a
+			// time (a wave reads whole rows), and the elements reach their
+			// threads through the exchange, element i of line l at
+			// (i / 16) 17 + i % 16.
+#pragma unroll
+			for (int step = 0; step < 4; ++step)
+			{
+				const int q = step * 256 + t;  // float4 index within the block's 16 lines
+				const int ll = q >> 6;         // line
+				const int i = (q & 63) * 4;    // first element
+				const int c = i - offset;
+				const int64_t sa = row_in[2 * ll];
+				const int64_t sb = row_in[2 * ll + 1];
+				float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+				float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+				if (c >= 0 && c < m)
+				{
+					if (sa >= 0)
+					{
+						const float g = row_scale[2 * ll];
+						a = load_real4<KIND>(in, sa + c);
+						a = make_float4(a.x * g, a.y * g, a.z * g, a.w * g);
+					}
+					if (sb >= 0)
+					{
+						const float g = row_scale[2 * ll + 1];
+						b = load_real4<KIND>(in, sb + c);
+						b = make_float4(b.x * g, b.y * g, b.z * g, b.w * g);
+					}
+				}
+				float2 *xl = exchange + ll * fft256_line_stride + (i >> 4) * 17 + (i & 15);
+				xl[0] = make_float2(a.x, b.x);
+				xl[1] = make_float2(a.y, b.y);
+				xl[2] = make_float2(a.z, b.z);
+				xl[3] = make_float2(a.w, b.w);
+			}
+			__syncthreads();
+#pragma unroll
+			for (int k = 0; k < 16; ++k)
+			{
+				v[k] = x[k * 17 + j];
+			}
+			__syncthreads();
+		}
+		else
+		{
+			const int64_t src_a = row_in[2 * l];
+			const int64_t src_b = row_in[2 * l + 1];
+			const float scale_a = row_scale[2 * l];
+			const float scale_b = row_scale[2 * l + 1];
+#pragma unroll
+			for (int k = 0; k < 16; ++k)
+			{
+				const int c = j + 16 * k - offset;
+				const bool inside = c >= 0 && c < m;
+				const float a =
+					src_a >= 0 && inside ? load_real<KIND>(in, src_a + c * in_rows.s_elem) * scale_a : 0.0f;
+				const float b =
+					src_b >= 0 && inside ? load_real<KIND>(in, src_b + c * in_rows.s_elem) * scale_b : 0.0f;
+				v[k] = make_float2(a, b);
+			}
+		}
+		fft256_four_step(v, x, twl, j, -1.0f);
+		__syncthreads();
+#pragma unroll
+		for (int k2 = 0; k2 < 16; ++k2)
+		{
+			x[k2 * 17 + j] = v[k2]; // Z[j + 16 k2]
+		}
+		__syncthreads();
+		const int64_t dst_a = row_out[2 * l];
+		const int64_t dst_b = row_out[2 * l + 1];
+#pragma unroll
+		for (int q = 0; q < 9; ++q)
+		{
+			const int k = j + 16 * q;
+			if (k < h)
+			{
+				const int km = (n - k) & (n - 1);
+				const float2 zk = x[q * 17 + j];
+				const float2 zm = cconj(x[(km >> 4) * 17 + (km & 15)]);
+				if (dst_a >= 0)
+				{
+					const float2 sum = cadd(zk, zm);
+					out[dst_a + k * out_rows.s_elem] = make_float2(0.5f * sum.x, 0.5f * sum.y);
+				}
+				if (dst_b >= 0)
+				{
+					// (zk - zm) / (2 i)
+					const float2 diff = csub(zk, zm);
+					out[dst_b + k * out_rows.s_elem] = make_float2(0.5f * diff.y, -0.5f * diff.x);
+				}
+			}
+		}
+		__syncthreads();
+	}
+}
+
 // Half-spectrum rows, two at a time, to real rows, scaled.
 template <int N>
 __global__ void irfft_rows_kernel(const float2 *src, line_set src_rows, float *dst, line_set dst_rows,
@@ -557,6 +746,40 @@ void launch_rfft_rows(int kind, const void *in, const line_set &in_rows, int m, 
 {
 	const fft_plan plan = make_fft_plan(n);
 	const float2 *tw = twiddles(n);
+	if (n == 256)
+	{
+		const unsigned blocks = xcd_grid(grid_for(out_rows.count, 2 * fft256_lines));
+		// Rows take loads of 4 elements when every run of 4 elements is
+		// aligned and contiguous.
+		const std::uintptr_t element = kind == real_f32 ? 4 : 2;
+		const bool aligned = in_rows.s_elem == 1 && m % 4 == 0 && (n / 2 - m / 2) % 4 == 0 &&
+			in_rows.s_outer % 4 == 0 && in_rows.s_middle % 4 == 0 && in_rows.s_inner % 4 == 0 &&
+			reinterpret_cast<std::uintptr_t>(in) % (4 * element) == 0;
+		auto launch = [&](auto kind_constant, auto vec_constant)
+		{
+			rfft256_rows_kernel<decltype(kind_constant)::value, decltype(vec_constant)::value>
+				<<<blocks, 256, 0, stream>>>(in, in_rows, m, scales, scale_stride, out, out_rows, tw);
+		};
+		using vec = std::true_type;
+		using scalar = std::false_type;
+		switch (kind)
+		{
+		case real_f32:
+			aligned ? launch(std::integral_constant<int, real_f32>(), vec()) :
+				launch(std::integral_constant<int, real_f32>(), scalar());
+			break;
+		case real_bf16:
+			aligned ? launch(std::integral_constant<int, real_bf16>(), vec()) :
+				launch(std::integral_constant<int, real_bf16>(), scalar());
+			break;
+		default:
+			aligned ? launch(std::integral_constant<int, real_f16>(), vec()) :
+				launch(std::integral_constant<int, real_f16>(), scalar());
+			break;
+		}
+		REXLIB_HIP_CHECK(hipGetLastError());
+		return;
+	}
 	const int pairs = std::max(1, lines_per_group(static_cast<int>(n)) / 2);
 	// Two buffers, the twiddles, then per row two offsets and a scale.
 	const std::size_t shared = sizeof(float2) * (2 * pairs * (n + 1) + n) + 2 * pairs * (2 * sizeof(int64_t) + 4);

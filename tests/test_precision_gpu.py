@@ -90,6 +90,34 @@ def test_fused_images_at_256_equal_rfft2_then_insertion(schedule, monkeypatch):
     assert oracle.rel_l2(den.cpu().numpy(), ref[1]) <= 1e-4
 
 
+@pytest.mark.parametrize("dtype", ["bf16", "fp16"])
+def test_reduced_precision_staging_at_256_equals_the_dequantised_images(dtype):
+    # The staged images, widened to float32 on chip (with the per-image
+    # scale for fp16), must give what the same widened images give through
+    # rfft2 and insert_ctf_slices in float32.
+    stack = make_stack(count=24, m=128, n=256, seed=7)
+    n, big = 256, 64
+    staged, scales = precision.stage_images(stack.images, dtype)
+    num = torch.zeros((big, big, big // 2 + 1), dtype=torch.complex64, device="cuda")
+    den = torch.zeros((big, big, big // 2 + 1), dtype=torch.float32, device="cuda")
+    em.insert_ctf_images(num, den, staged, to_device(stack.transforms), to_device(stack.ctf),
+                         torch.ones(n // 2 + 1, device="cuda"), em.GriddingWindow.linear(), OPTICS, 1.5,
+                         image_scales=scales, padded=n)
+    widened = staged.float() if scales is None else staged.float() * scales[:, None, None]
+    images = torch.zeros((widened.shape[0], n, n), device="cuda")
+    o = n // 2 - 128 // 2
+    images[:, o:o + 128, o:o + 128] = widened
+    slices = rexlib_amd.rfft2(images)
+    num2 = torch.zeros_like(num)
+    den2 = torch.zeros_like(den)
+    em.insert_ctf_slices(num2, den2, slices, to_device(stack.transforms), to_device(stack.ctf),
+                         torch.ones(n // 2 + 1, device="cuda"), em.GriddingWindow.linear(), OPTICS, 1.5,
+                         numerator_out=num2, denominator_out=den2)
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(num.cpu().numpy(), num2.cpu().numpy()) < 1e-6
+    assert oracle.rel_l2(den.cpu().numpy(), den2.cpu().numpy()) < 1e-6
+
+
 def test_float32_staging_with_padding_matches_the_oracle():
     stack = make_stack()
     n = big = 64
