@@ -457,14 +457,25 @@ __global__ __launch_bounds__(256) void fft256_lines_kernel(const float2 *src, li
 		{
 			v[k] = live ? in[(j + 16 * k) * src_lines.s_elem] : make_float2(0.0f, 0.0f);
 		}
-		fft256_four_step(v, exchange + l * fft256_line_stride, twl, j, sign);
+		float2 *x = exchange + l * fft256_line_stride;
+		fft256_four_step(v, x, twl, j, sign);
+		// The outputs go back to LDS (each thread to slots only it reads),
+		// so the stores loop over them without holding all 16.
+		__syncthreads();
+#pragma unroll
+		for (int k2 = 0; k2 < 16; ++k2)
+		{
+			x[k2 * 17 + j] = v[k2];
+		}
 		if (live)
 		{
-			float2 *out = dst + line_dst[l];
-#pragma unroll
+			float2 *out = dst + line_dst[l] + j * dst_lines.s_elem;
+			const int64_t step = 16 * dst_lines.s_elem;
+#pragma unroll 4
 			for (int k2 = 0; k2 < 16; ++k2)
 			{
-				out[(j + 16 * k2) * dst_lines.s_elem] = make_float2(v[k2].x * scale, v[k2].y * scale);
+				const float2 y = x[k2 * 17 + j];
+				out[k2 * step] = make_float2(y.x * scale, y.y * scale);
 			}
 		}
 		__syncthreads();
