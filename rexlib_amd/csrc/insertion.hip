@@ -763,16 +763,17 @@ __global__ __launch_bounds__(256) void prepare_kernel(insertion_params p, slice_
 // zeroed afterwards where prepare_value rejects it. The loads of several rows
 // can then be in flight together. The arithmetic is prepare_value's, in the
 // same order.
+// `radial` is the slice's radial weights, `y_table` its row phases.
 template <int VARIANT>
 __device__ inline prepared_sample prepare_unbranched(const insertion_params &p, const slice_entry &e,
-	const float2 *table, float2 x_phase, int row, int column, float2 x)
+	const float *radial, const float2 *y_table, float2 x_phase, int row, int column, float2 x)
 {
 	const int n = p.n;
 	const int ky = row <= n / 2 ? row : row - n;
 	const int shell = shell_of(column, ky);
 	const bool nyquist = (n % 2 == 0) && (row == n / 2 || column == n / 2);
-	const float r = p.radial[e.radial_offset + min(shell, max(e.last_shell, 0)) * p.radial_s0];
-	const float2 y_phase = table[p.h + row];
+	const float r = radial[min(shell, max(e.last_shell, 0)) * p.radial_s0];
+	const float2 y_phase = y_table[row];
 	float numerator_weight = 1.0f;
 	float denominator_weight = 1.0f;
 	if (VARIANT == variant_weighted)
@@ -852,12 +853,14 @@ __global__ __launch_bounds__(256) void columns_prepare256_kernel(insertion_param
 		}
 		__syncthreads();
 		const bool live = l < lines;
-		const float2 *in = p.slices + line_src[l];
+		// Elements j + 16 k of the column, by pointer steps.
+		const float2 *in = p.slices + line_src[l] + j * p.slice_s0;
+		const int64_t in_step = 16 * p.slice_s0;
 		float2 v[16];
 #pragma unroll
 		for (int k = 0; k < 16; ++k)
 		{
-			v[k] = live ? in[(j + 16 * k) * p.slice_s0] : make_float2(0.0f, 0.0f);
+			v[k] = live ? in[k * in_step] : make_float2(0.0f, 0.0f);
 		}
 		float2 *x = exchange + l * fft256_line_stride;
 		fft256_four_step(v, x, twl, j, -1.0f);
@@ -876,12 +879,13 @@ __global__ __launch_bounds__(256) void columns_prepare256_kernel(insertion_param
 			const int column = line_column[l];
 			const float2 *table = p.tables + slice * static_cast<int64_t>(h + p.n);
 			const float2 x_phase = table[column];
-			prepared_sample *out = prepared + line_dst[l];
+			const float *radial = p.radial + e.radial_offset;
+			prepared_sample *out = prepared + line_dst[l] + j * h;
 #pragma unroll 4
 			for (int k2 = 0; k2 < 16; ++k2)
 			{
-				const int row = j + 16 * k2;
-				out[row * h] = prepare_unbranched<VARIANT>(p, e, table, x_phase, row, column, x[k2 * 17 + j]);
+				out[k2 * 16 * h] = prepare_unbranched<VARIANT>(p, e, radial, table + h, x_phase, j + 16 * k2, column,
+					x[k2 * 17 + j]);
 			}
 		}
 		__syncthreads();
