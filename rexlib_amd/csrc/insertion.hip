@@ -1148,8 +1148,10 @@ __device__ inline void add_sample(const insertion_params &p, const brick_box &bo
 // bound_rows of one slice and sign: the rows [r0, r1) whose samples k_x in
 // [first, widest] can reach the box [lower, upper]; r0 >= r1 when none.
 // Evaluated per lane, one slice per lane.
+// `inv_y` holds 1 / y_step_f per axis (0 where the step is 0); a product with
+// it stands for the division, the box being widened far beyond its rounding.
 __device__ inline void bound_rows(const insertion_params &p, const float *x_step, const float *y_step_f,
-	int last_shell, const float *lower, const float *upper, int sign_index, int &r0, int &r1)
+	const float *inv_y, int last_shell, const float *lower, const float *upper, int sign_index, int &r0, int &r1)
 {
 	r0 = 0;
 	r1 = 0;
@@ -1184,8 +1186,9 @@ __device__ inline void bound_rows(const insertion_params &p, const float *x_step
 		}
 		else
 		{
-			const float l = lo / y_step;
-			const float u = hi / y_step;
+			const float inv = sign * inv_y[a];
+			const float l = lo * inv;
+			const float u = hi * inv;
 			low = fmaxf(low, fminf(l, u));
 			high = fminf(high, fmaxf(l, u));
 		}
@@ -1202,10 +1205,11 @@ __device__ inline void bound_rows(const insertion_params &p, const float *x_step
 
 // One slice, one sign, into a brick, over the rows [r0, r1) that reach it;
 // the whole wavefront takes part.
+// `inv_x` holds 1 / x_step per axis (0 where the step is 0).
 template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
 __device__ void insert_slice_sign(const insertion_params &p, const float *x_step, const float *y_step_f,
-	int last_shell, const prepared_sample *slice, const brick_box &box, const float *lower, const float *upper, int sign_index,
-	int r0, int r1, brick_voxel *acc, int *row_table)
+	const float *inv_x, int last_shell, const prepared_sample *slice, const brick_box &box, const float *lower,
+	const float *upper, int sign_index, int r0, int r1, brick_voxel *acc, int *row_table)
 {
 	const int lane = threadIdx.x & 63;
 	const int n = p.n;
@@ -1223,7 +1227,7 @@ __device__ void insert_slice_sign(const insertion_params &p, const float *x_step
 	{
 		step[a] = sign * x_step[a];
 		y_step[a] = sign * y_step_f[a];
-		inv_step[a] = step[a] == 0.0f ? 0.0f : 1.0f / step[a];
+		inv_step[a] = sign * inv_x[a];
 	}
 	for (int rc = r0; rc < r1; rc += 64)
 	{
@@ -1587,14 +1591,24 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 						centre_norm <= spread * 1.001f + half_diagonal + 1e-3f;
 				}
 			}
-			// bound_rows of both signs, one slice per lane.
+			// bound_rows of both signs, one slice per lane, and the
+			// reciprocal steps of the frame (shared by both signs).
 			int r0_plus = 0, r1_plus = 0, r0_minus = 0, r1_minus = 0;
+			float ix0 = 0.0f, ix1 = 0.0f, ix2 = 0.0f;
 			if (near)
 			{
 				const float xs[3] = {x0, x1, x2};
 				const float ys[3] = {y0, y1, y2};
-				bound_rows(p, xs, ys, last_shell, lower, upper, 0, r0_plus, r1_plus);
-				bound_rows(p, xs, ys, last_shell, lower, upper, 1, r0_minus, r1_minus);
+				float iy[3];
+				for (int a = 0; a < 3; ++a)
+				{
+					iy[a] = ys[a] == 0.0f ? 0.0f : 1.0f / ys[a];
+				}
+				ix0 = x0 == 0.0f ? 0.0f : 1.0f / x0;
+				ix1 = x1 == 0.0f ? 0.0f : 1.0f / x1;
+				ix2 = x2 == 0.0f ? 0.0f : 1.0f / x2;
+				bound_rows(p, xs, ys, iy, last_shell, lower, upper, 0, r0_plus, r1_plus);
+				bound_rows(p, xs, ys, iy, last_shell, lower, upper, 1, r0_minus, r1_minus);
 				near = r0_plus < r1_plus || r0_minus < r1_minus;
 			}
 			unsigned long long mask = __ballot(near);
@@ -1607,6 +1621,7 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 				// The hit slice's frame, read from the lane that tested it.
 				const float x_step[3] = {lane_value(x0, bit), lane_value(x1, bit), lane_value(x2, bit)};
 				const float y_step[3] = {lane_value(y0, bit), lane_value(y1, bit), lane_value(y2, bit)};
+				const float inv_x[3] = {lane_value(ix0, bit), lane_value(ix1, bit), lane_value(ix2, bit)};
 				const int shell = __builtin_amdgcn_readlane(last_shell, bit);
 				const bool exact = __builtin_amdgcn_readlane(lattice_free, bit) != 0;
 				for (int sign_index = 0; sign_index < 2; ++sign_index)
@@ -1619,13 +1634,13 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 					}
 					if (exact)
 					{
-						insert_slice_sign<PROFILE, ODD, false, EY, EZ>(p, x_step, y_step, shell, slice, box, lower,
-							upper, sign_index, r0, r1, acc, row_table);
+						insert_slice_sign<PROFILE, ODD, false, EY, EZ>(p, x_step, y_step, inv_x, shell, slice, box,
+							lower, upper, sign_index, r0, r1, acc, row_table);
 					}
 					else
 					{
-						insert_slice_sign<PROFILE, ODD, true, EY, EZ>(p, x_step, y_step, shell, slice, box, lower,
-							upper, sign_index, r0, r1, acc, row_table);
+						insert_slice_sign<PROFILE, ODD, true, EY, EZ>(p, x_step, y_step, inv_x, shell, slice, box,
+							lower, upper, sign_index, r0, r1, acc, row_table);
 					}
 				}
 			}
