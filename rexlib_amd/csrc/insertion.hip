@@ -1599,14 +1599,16 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 			{
 				const float xs[3] = {x0, x1, x2};
 				const float ys[3] = {y0, y1, y2};
+				// The hardware reciprocal (within 1 ulp): the bounds it feeds
+				// are widened by 0.01 voxel, far beyond its error.
 				float iy[3];
 				for (int a = 0; a < 3; ++a)
 				{
-					iy[a] = ys[a] == 0.0f ? 0.0f : 1.0f / ys[a];
+					iy[a] = ys[a] == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(ys[a]);
 				}
-				ix0 = x0 == 0.0f ? 0.0f : 1.0f / x0;
-				ix1 = x1 == 0.0f ? 0.0f : 1.0f / x1;
-				ix2 = x2 == 0.0f ? 0.0f : 1.0f / x2;
+				ix0 = x0 == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(x0);
+				ix1 = x1 == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(x1);
+				ix2 = x2 == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(x2);
 				bound_rows(p, xs, ys, iy, last_shell, lower, upper, 0, r0_plus, r1_plus);
 				bound_rows(p, xs, ys, iy, last_shell, lower, upper, 1, r0_minus, r1_minus);
 				near = r0_plus < r1_plus || r0_minus < r1_minus;
