@@ -91,139 +91,6 @@ const float2 *twiddles(int64_t n)
 namespace
 {
 
-template <int R>
-__device__ inline void stage(const float2 *a, float2 *b, int lines, int n, int line_stride, int ns,
-	const float2 *tw, float sign)
-{
-	const int m = n / R;
-	const int twiddle_step = n / (ns * R);
-	for (int t = threadIdx.x; t < lines * m; t += blockDim.x)
-	{
-		const int line = t / m;
-		const int j = t - line * m;
-		const int k = j % ns;
-		const float2 *src = a + line * line_stride;
-		float2 v[R];
-#pragma unroll
-		for (int r = 0; r < R; ++r)
-		{
-			v[r] = src[j + r * m];
-		}
-#pragma unroll
-		for (int r = 1; r < R; ++r)
-		{
-			float2 w = tw[k * r * twiddle_step];
-			w.y *= -sign; // the table holds exp(-2 pi i m / n)
-			v[r] = cmul(v[r], w);
-		}
-		dft<R>(v, sign);
-		const int d = (j / ns) * ns * R + k;
-		float2 *dst = b + line * line_stride;
-#pragma unroll
-		for (int r = 0; r < R; ++r)
-		{
-			dst[d + r * ns] = v[r];
-		}
-	}
-}
-
-// Transform `lines` lines of LDS buffer a (line stride line_stride); returns
-// the buffer holding the result. sign -1: forward, +1: inverse (unscaled).
-__device__ float2 *block_fft(float2 *a, float2 *b, int lines, int line_stride, const fft_plan &plan,
-	const float2 *tw, float sign)
-{
-	int ns = 1;
-	const int n = plan.n;
-	for (int st = 0; st < plan.stages; ++st)
-	{
-		const int r = plan.radix[st];
-		switch (r)
-		{
-		case 4: stage<4>(a, b, lines, n, line_stride, ns, tw, sign); break;
-		case 2: stage<2>(a, b, lines, n, line_stride, ns, tw, sign); break;
-		case 3: stage<3>(a, b, lines, n, line_stride, ns, tw, sign); break;
-		default: stage<5>(a, b, lines, n, line_stride, ns, tw, sign); break;
-		}
-		__syncthreads();
-		float2 *swap = a;
-		a = b;
-		b = swap;
-		ns *= r;
-	}
-	return a;
-}
-
-// The same transform for a power-of-two length N known at compile time:
-// radix 4 while it divides, then one radix 2 (the order make_fft_plan gives),
-// with every index computed by shifts and masks.
-template <int N, int R, int NS>
-__device__ inline void stage_fixed(const float2 *a, float2 *b, int lines, int line_stride, const float2 *tw,
-	float sign)
-{
-	constexpr int m = N / R;
-	constexpr int twiddle_step = N / (NS * R);
-	for (int t = threadIdx.x; t < lines * m; t += blockDim.x)
-	{
-		const int line = t / m;
-		const int j = t % m;
-		const int k = j % NS;
-		const float2 *src = a + line * line_stride;
-		float2 v[R];
-#pragma unroll
-		for (int r = 0; r < R; ++r)
-		{
-			v[r] = src[j + r * m];
-		}
-#pragma unroll
-		for (int r = 1; r < R; ++r)
-		{
-			float2 w = tw[k * r * twiddle_step];
-			w.y *= -sign; // the table holds exp(-2 pi i m / n)
-			v[r] = cmul(v[r], w);
-		}
-		dft<R>(v, sign);
-		const int d = (j / NS) * NS * R + k;
-		float2 *dst = b + line * line_stride;
-#pragma unroll
-		for (int r = 0; r < R; ++r)
-		{
-			dst[d + r * NS] = v[r];
-		}
-	}
-}
-
-template <int N, int NS = 1>
-__device__ inline float2 *block_fft_fixed(float2 *a, float2 *b, int lines, int line_stride, const float2 *tw,
-	float sign)
-{
-	if constexpr (NS >= N)
-	{
-		return a;
-	}
-	else
-	{
-		constexpr int R = (N / NS) % 4 == 0 ? 4 : 2;
-		stage_fixed<N, R, NS>(a, b, lines, line_stride, tw, sign);
-		__syncthreads();
-		return block_fft_fixed<N, NS * R>(b, a, lines, line_stride, tw, sign);
-	}
-}
-
-// The fixed form for N > 0, the planned one otherwise.
-template <int N>
-__device__ inline float2 *transform(float2 *a, float2 *b, int lines, int line_stride, const fft_plan &plan,
-	const float2 *tw, float sign)
-{
-	if constexpr (N > 0)
-	{
-		return block_fft_fixed<N>(a, b, lines, line_stride, tw, sign);
-	}
-	else
-	{
-		return block_fft(a, b, lines, line_stride, plan, tw, sign);
-	}
-}
-
 template <int KIND>
 __device__ inline float load_real(const void *base, int64_t index);
 
@@ -728,14 +595,14 @@ unsigned grid_for(int64_t work, int64_t per_block)
 	return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(blocks, 1 << 16)));
 }
 
+} // anonymous namespace
+
 int lines_per_group(int n)
 {
 	// About 2048 complex elements (16 KiB) per LDS buffer: small blocks keep
 	// more wavefronts per CU (measured faster than 4096 or 8192 at n = 256).
 	return std::max(1, std::min(8, 2048 / std::max(n, 1)));
 }
-
-} // anonymous namespace
 
 // Calls f.template operator()<N>() with N the length when it is a power of
 // two from 64 to 512, else 0 (the planned transform).

@@ -46,6 +46,9 @@ fft_plan make_fft_plan(int64_t n);
 // device and length.
 const float2 *twiddles(int64_t n);
 
+// Lines per block of the LDS transforms of length n.
+int lines_per_group(int n);
+
 // Stage callers: the element type of real lines.
 enum real_kind : int
 {

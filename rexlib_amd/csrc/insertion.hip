@@ -892,6 +892,80 @@ __global__ __launch_bounds__(256) void columns_prepare256_kernel(insertion_param
 	}
 }
 
+// The same fusion for the other lengths the FFT kernels take: a block
+// stages `group` columns of the chunk's slices in LDS, transforms them with
+// the Stockham stages (compile-time for N > 0, planned for N = 0), and
+// prepares every element from LDS. The lines are (slice j of the chunk,
+// column), numbered j columns + column, where columns is n/2 for even n
+// (the Nyquist column, never inserted, is skipped) and n/2 + 1 for odd n.
+template <int VARIANT, int N>
+__global__ __launch_bounds__(256) void columns_prepare_kernel(insertion_params p, slice_chunk chunk, fft_plan plan,
+	const float2 *tw, int group, prepared_sample *prepared)
+{
+	extern __shared__ float2 lds[];
+	const int n = N > 0 ? N : plan.n;
+	const int stride = n + 1;
+	float2 *a = lds;
+	float2 *b = lds + group * stride;
+	float2 *twl = lds + 2 * group * stride;
+	int64_t *line_slice = reinterpret_cast<int64_t *>(twl + n);
+	int64_t *line_src = line_slice + group;
+	int64_t *line_dst = line_src + group;
+	int *line_column = reinterpret_cast<int *>(line_dst + group);
+	for (int i = threadIdx.x; i < n; i += blockDim.x)
+	{
+		twl[i] = tw[i];
+	}
+	const int h = p.h;
+	const int columns = n % 2 == 0 ? n / 2 : h;
+	const int64_t elements = static_cast<int64_t>(n) * h;
+	const int64_t total = p.map.kept_count * chunk.count * columns;
+	for (int64_t first = xcd_block(blockIdx.x, gridDim.x) * static_cast<int64_t>(group); first < total;
+		first += static_cast<int64_t>(gridDim.x) * group)
+	{
+		const int lines = static_cast<int>(total - first < group ? total - first : group);
+		for (int q = threadIdx.x; q < lines; q += blockDim.x)
+		{
+			const int64_t line = first + q;
+			const int64_t jj = line / columns;
+			const int column = static_cast<int>(line - jj * columns);
+			const int64_t kept = jj / chunk.count;
+			const int64_t slice = kept * p.map.reduced_count + chunk.begin + jj % chunk.count;
+			line_slice[q] = slice;
+			line_src[q] = p.entries[slice].slice_offset + column * p.slice_s1;
+			line_dst[q] = jj * elements + column;
+			line_column[q] = column;
+		}
+		__syncthreads();
+		for (int t = threadIdx.x; t < group * n; t += blockDim.x)
+		{
+			const int e = t / group;
+			const int l = t - e * group;
+			if (l < lines)
+			{
+				a[l * stride + e] = p.slices[line_src[l] + e * p.slice_s0];
+			}
+		}
+		__syncthreads();
+		const float2 *z = transform<N>(a, b, lines, stride, plan, twl, -1.0f);
+		for (int t = threadIdx.x; t < group * n; t += blockDim.x)
+		{
+			const int e = t / group;
+			const int l = t - e * group;
+			if (l < lines)
+			{
+				const int64_t slice = line_slice[l];
+				const slice_entry &entry = p.entries[slice];
+				const float2 *table = p.tables + slice * static_cast<int64_t>(h + n);
+				const int column = line_column[l];
+				prepared[line_dst[l] + static_cast<int64_t>(e) * h] = prepare_unbranched<VARIANT>(p, entry,
+					p.radial + entry.radial_offset, table + h, table[column], e, column, z[l * stride + e]);
+			}
+		}
+		__syncthreads();
+	}
+}
+
 // Box and triangle windows on an even N: per sample, the first tap's brick
 // index, its volume phase (-1)^(x+y+z) (a wrapped index keeps its parity)
 // and the in-brick bits of the taps; each tap is then a constant offset and a
@@ -1796,9 +1870,54 @@ void dispatch_prepare(const insertion_params &p, slice_chunk chunk, prepared_sam
 	}
 }
 
+template <int VARIANT>
+void launch_columns_prepare(const insertion_params &p, slice_chunk chunk, prepared_sample *prepared,
+	hipStream_t stream)
+{
+	const int n = p.n;
+	const int columns = n % 2 == 0 ? n / 2 : p.h;
+	const int64_t lines = p.map.kept_count * chunk.count * columns;
+	const fft_plan plan = make_fft_plan(n);
+	const float2 *tw = twiddles(n);
+	const int group = lines_per_group(n);
+	const std::size_t shared = sizeof(float2) * (2 * group * (n + 1) + n) + group * (3 * sizeof(int64_t) + sizeof(int));
+	const unsigned grid = xcd_grid(static_cast<unsigned>(std::min<int64_t>((lines + group - 1) / group, 1 << 20)));
+	switch (n)
+	{
+	case 64:
+		columns_prepare_kernel<VARIANT, 64><<<grid, 256, shared, stream>>>(p, chunk, plan, tw, group, prepared);
+		break;
+	case 128:
+		columns_prepare_kernel<VARIANT, 128><<<grid, 256, shared, stream>>>(p, chunk, plan, tw, group, prepared);
+		break;
+	case 512:
+		columns_prepare_kernel<VARIANT, 512><<<grid, 256, shared, stream>>>(p, chunk, plan, tw, group, prepared);
+		break;
+	default:
+		columns_prepare_kernel<VARIANT, 0><<<grid, 256, shared, stream>>>(p, chunk, plan, tw, group, prepared);
+		break;
+	}
+}
+
 void dispatch_columns_prepare(const insertion_params &p, slice_chunk chunk, prepared_sample *prepared,
 	hipStream_t stream)
 {
+	if (p.n != 256)
+	{
+		switch (p.variant)
+		{
+		case variant_plain:
+			launch_columns_prepare<variant_plain>(p, chunk, prepared, stream);
+			break;
+		case variant_weighted:
+			launch_columns_prepare<variant_weighted>(p, chunk, prepared, stream);
+			break;
+		default:
+			launch_columns_prepare<variant_ctf>(p, chunk, prepared, stream);
+			break;
+		}
+		return;
+	}
 	const int64_t lines = p.map.kept_count * chunk.count * (p.n / 2);
 	const unsigned grid = xcd_grid(static_cast<unsigned>(
 		std::min<int64_t>((lines + fft256_lines - 1) / fft256_lines, 1 << 20)));
@@ -1998,9 +2117,9 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 	// The brick schedule needs the window to span at most a brick; wider
 	// Kaiser-Bessel windows take the scatter schedule.
 	const bool brick = schedule != schedule_scatter && p.taps <= 4 && p.map.kept_count <= 65535;
-	// Pending column transforms of 256-point slices run inside the
-	// preparation of each chunk; any others run here, in place.
-	const bool fused_columns = columns != nullptr && brick && p.n == 256;
+	// Pending column transforms run inside the preparation of each chunk on
+	// the brick schedule; on the scatter schedule they run here, in place.
+	const bool fused_columns = columns != nullptr && brick;
 	if (columns != nullptr && !fused_columns && columns->lines.count > 0)
 	{
 		launch_fft_lines(columns->spectra, columns->lines, columns->spectra, columns->lines, p.n, -1.0f, 1.0f,

@@ -29,8 +29,8 @@ enum insertion_schedule : int
 };
 
 // Slices whose rows are transformed but whose columns are not yet (the
-// fused image insertion): the insertion runs the column transforms, for
-// 256-point slices together with the preparation of the samples.
+// fused image insertion): the insertion runs the column transforms, on the
+// brick schedule together with the preparation of the samples.
 struct pending_columns
 {
 	float2 *spectra;
