@@ -286,50 +286,124 @@ struct tile_lds
 
 constexpr int mlp_waves = 4;
 
-// out[i] = MLP(features[i]) for i < count.
-__global__ __launch_bounds__(64 * mlp_waves) void mlp_forward_kernel(const __half *features, const _Float16 *params_h,
+// out[i] = MLP(features[i]) for i < count, in transposed form: each layer
+// computes Y^T = W^T X^T on a tile of 16 samples, the 16 columns of the B
+// operand. The weights are the A operands and stay in registers for the
+// whole kernel. An MFMA's output fragment (lane l: rows 4 (l / 16) + j,
+// column l % 16) is laid out as the B operand of the next layer's k-tile, so
+// after bias, ReLU and the float16 rounding the activations never leave
+// registers. Lane l of A holds W^T[m0 + l % 16][k0 + 4 (l / 16) + j] =
+// W[k0 + 4 (l / 16) + j][m0 + l % 16].
+__global__ __launch_bounds__(256) void mlp_forward_kernel(const __half *features, const _Float16 *params_h,
 	const float *params_f, float *out, int64_t count)
 {
-	__shared__ mlp_lds s;
-	__shared__ tile_lds tiles[mlp_waves];
-	stage_parameters(s, params_h, params_f);
-	const int lane = threadIdx.x & 63;
-	const int wave = threadIdx.x >> 6;
-	tile_lds &t = tiles[wave];
-	const int64_t tiles_total = (count + 15) / 16;
-	for (int64_t tile = blockIdx.x * static_cast<int64_t>(mlp_waves) + wave; tile < tiles_total;
-		tile += static_cast<int64_t>(gridDim.x) * mlp_waves)
+	// Biases per (layer, m-tile, lane group): the 4 rows a lane's output holds.
+	__shared__ float4_t bias[3][4][4];
+	if (threadIdx.x < 48)
 	{
-		const int64_t row0 = tile * 16;
-		for (int e = lane; e < 16 * inputs; e += 64)
+		const int layer = threadIdx.x / 16;
+		const int mt = (threadIdx.x / 4) % 4;
+		const int g = threadIdx.x % 4;
+		const float *b = params_f + off_b1 + layer * hidden + mt * 16 + 4 * g;
+		bias[layer][mt][g] = float4_t{b[0], b[1], b[2], b[3]};
+	}
+	__syncthreads();
+	const int lane = threadIdx.x & 63;
+	const int g = lane >> 4;
+	const int c = lane & 15;
+	half4_t w1[4];
+	half4_t w2[4][4];
+	half4_t w3[4][4];
+	half4_t w4[4];
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
+	{
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt)
 		{
-			const int64_t r = row0 + e / inputs;
-			t.x[e] = r < count ? static_cast<_Float16>(__half2float(features[r * inputs + e % inputs])) : _Float16(0);
-		}
-		wave_order();
-		dense_tile<inputs, true>(t.x, s.w + off_w1, s.b + (off_b1 - off_b1), t.h1, lane);
-		wave_order();
-		dense_tile<hidden, true>(t.h1, s.w + off_w2, s.b + (off_b2 - off_b1), t.h2, lane);
-		wave_order();
-		dense_tile<hidden, true>(t.h2, s.w + off_w3, s.b + (off_b3 - off_b1), t.h3, lane);
-		wave_order();
-		float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-		for (int kt = 0; kt < hidden / 16; ++kt)
-		{
-			acc = mfma16(load_a(t.h3, hidden, 0, kt * 16, lane), load_b(s.w + off_w4, 16, kt * 16, 0, lane), acc);
-		}
-		if ((lane & 15) == 0)
-		{
-			for (int j = 0; j < 4; ++j)
+			w1[mt][j] = params_h[off_w1 + (4 * g + j) * hidden + mt * 16 + c];
+#pragma unroll
+			for (int kt = 0; kt < 4; ++kt)
 			{
-				const int64_t r = row0 + 4 * (lane >> 4) + j;
-				if (r < count)
-				{
-					out[r] = acc[j] + s.b[off_b4 - off_b1];
-				}
+				w2[mt][kt][j] = params_h[off_w2 + (kt * 16 + 4 * g + j) * hidden + mt * 16 + c];
+				w3[mt][kt][j] = params_h[off_w3 + (kt * 16 + 4 * g + j) * hidden + mt * 16 + c];
 			}
 		}
-		wave_order();
+#pragma unroll
+		for (int kt = 0; kt < 4; ++kt)
+		{
+			w4[kt][j] = params_h[off_w4 + (kt * 16 + 4 * g + j) * 16 + c];
+		}
+	}
+	const float b4 = params_f[off_b4];
+	const int64_t tiles = (count + 15) / 16;
+	const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / 64);
+	for (int64_t tile = static_cast<int64_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6); tile < tiles;
+		tile += waves)
+	{
+		const int64_t row = tile * 16 + c;
+		// B fragment of X^T: B[4 g + j][c] = X[row][4 g + j], 8 bytes per lane.
+		half4_t x = {0, 0, 0, 0};
+		if (row < count)
+		{
+			x = *reinterpret_cast<const half4_t *>(features + row * inputs + 4 * g);
+		}
+		half4_t h1[4];
+		half4_t h2[4];
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt)
+		{
+			const float4_t acc = mfma16(w1[mt], x, float4_t{0.0f, 0.0f, 0.0f, 0.0f});
+			const float4_t b = bias[0][mt][g];
+#pragma unroll
+			for (int j = 0; j < 4; ++j)
+			{
+				h1[mt][j] = static_cast<_Float16>(fmaxf(acc[j] + b[j], 0.0f));
+			}
+		}
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt)
+		{
+			float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+			for (int kt = 0; kt < 4; ++kt)
+			{
+				acc = mfma16(w2[mt][kt], h1[kt], acc);
+			}
+			const float4_t b = bias[1][mt][g];
+#pragma unroll
+			for (int j = 0; j < 4; ++j)
+			{
+				h2[mt][j] = static_cast<_Float16>(fmaxf(acc[j] + b[j], 0.0f));
+			}
+		}
+#pragma unroll
+		for (int mt = 0; mt < 4; ++mt)
+		{
+			float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+			for (int kt = 0; kt < 4; ++kt)
+			{
+				acc = mfma16(w3[mt][kt], h2[kt], acc);
+			}
+			const float4_t b = bias[2][mt][g];
+#pragma unroll
+			for (int j = 0; j < 4; ++j)
+			{
+				h1[mt][j] = static_cast<_Float16>(fmaxf(acc[j] + b[j], 0.0f)); // layer 3, reusing h1
+			}
+		}
+		float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+		for (int kt = 0; kt < 4; ++kt)
+		{
+			acc = mfma16(w4[kt], h1[kt], acc);
+		}
+		// Output row 0 (the one used column of W4) sits in lanes 0..15.
+		if (g == 0 && row < count)
+		{
+			out[row] = acc[0] + b4;
+		}
 	}
 }
 
@@ -576,7 +650,7 @@ void launch_features(const float *den, void *features, int big, hipStream_t stre
 void launch_mlp_forward(const void *features, const void *params_h, const float *params_f, float *out, int64_t count,
 	hipStream_t stream)
 {
-	mlp_forward_kernel<<<blocks_of((count + 15) / 16, mlp_waves), 64 * mlp_waves, 0, stream>>>(
+	mlp_forward_kernel<<<blocks_of((count + 15) / 16, 4 * 32), 256, 0, stream>>>(
 		static_cast<const __half *>(features), static_cast<const _Float16 *>(params_h), params_f, out, count);
 	REXLIB_HIP_CHECK(hipGetLastError());
 }
