@@ -34,47 +34,63 @@ namespace rexlib_hip
 namespace
 {
 
-// The stored index of full-spectrum frequency (x, y, z), each taken modulo N.
-__device__ inline int64_t hermitian_index(int x, int y, int z, int big, int half)
+// The stored index of full-spectrum frequency (x, y, z), each taken modulo
+// N, for x in [-1, N) and y, z in [-N, 2N) (the neighbours of a stored
+// voxel): frequencies with x >= N/2 + 1 are read at their Hermitian mirror
+// (-x, -y, -z).
+__device__ inline int64_t neighbour_index(int x, int y, int z, int big, int half)
 {
-	x = ((x % big) + big) % big;
-	y = ((y % big) + big) % big;
-	z = ((z % big) + big) % big;
+	x = x < 0 ? x + big : x;
+	y = y < 0 ? y + big : (y >= big ? y - big : y);
+	z = z < 0 ? z + big : (z >= big ? z - big : z);
 	if (x >= half)
 	{
 		x = big - x;
-		y = (big - y) % big;
-		z = (big - z) % big;
+		y = y == 0 ? 0 : big - y;
+		z = z == 0 ? 0 : big - z;
 	}
 	return (static_cast<int64_t>(z) * big + y) * half + x;
 }
 
-__global__ void compensation_pass_kernel(const float *den, const float *w_in, float *w_out, int big, float floor_value)
+// Threads over a (ceil(N half / blockDim), N) grid: (x, y) of plane z.
+__device__ inline bool voxel_of_thread(int big, int half, int &x, int &y, int &z)
+{
+	const int xy = blockIdx.x * blockDim.x + threadIdx.x;
+	if (xy >= big * half)
+	{
+		return false;
+	}
+	y = xy / half;
+	x = xy - y * half;
+	z = blockIdx.y;
+	return true;
+}
+
+__global__ __launch_bounds__(256) void compensation_pass_kernel(const float *den, const float *w_in, float *w_out,
+	int big, float floor_value)
 {
 	const int half = big / 2 + 1;
-	const int64_t count = static_cast<int64_t>(big) * big * half;
 	const float t[3] = {1.0f / 6.0f, 2.0f / 3.0f, 1.0f / 6.0f};
-	for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < count;
-		i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+	int x, y, z;
+	if (!voxel_of_thread(big, half, x, y, z))
 	{
-		const int x = static_cast<int>(i % half);
-		const int y = static_cast<int>((i / half) % big);
-		const int z = static_cast<int>(i / (static_cast<int64_t>(half) * big));
-		float c = 0.0f;
-		for (int dz = -1; dz <= 1; ++dz)
+		return;
+	}
+	const int64_t i = (static_cast<int64_t>(z) * big + y) * half + x;
+	float c = 0.0f;
+	for (int dz = -1; dz <= 1; ++dz)
+	{
+		for (int dy = -1; dy <= 1; ++dy)
 		{
-			for (int dy = -1; dy <= 1; ++dy)
+			for (int dx = -1; dx <= 1; ++dx)
 			{
-				for (int dx = -1; dx <= 1; ++dx)
-				{
-					const int64_t j = hermitian_index(x + dx, y + dy, z + dz, big, half);
-					c += t[dz + 1] * t[dy + 1] * t[dx + 1] * w_in[j] * den[j];
-				}
+				const int64_t j = neighbour_index(x + dx, y + dy, z + dz, big, half);
+				c += t[dz + 1] * t[dy + 1] * t[dx + 1] * w_in[j] * den[j];
 			}
 		}
-		const float w = w_in[i];
-		w_out[i] = den[i] > 0.0f ? w / fmaxf(c, floor_value) : 0.0f;
 	}
+	const float w = w_in[i];
+	w_out[i] = den[i] > 0.0f ? w / fmaxf(c, floor_value) : 0.0f;
 }
 
 __global__ void fill_kernel(float *out, int64_t count, float value)
@@ -86,51 +102,67 @@ __global__ void fill_kernel(float *out, int64_t count, float value)
 	}
 }
 
-// Features of every voxel, as 16 float16 per voxel (the last 6 zero).
-__global__ void features_kernel(const float *den, __half *features, int big)
+// Features of every voxel, as 16 float16 per voxel (the last 6 zero); the
+// 26 neighbours are read once and the 32 bytes stored as two 16-byte words.
+__global__ __launch_bounds__(256) void features_kernel(const float *den, __half *features, int big)
 {
 	const int half = big / 2 + 1;
-	const int64_t count = static_cast<int64_t>(big) * big * half;
-	for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < count;
-		i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+	int x, sy, sz;
+	if (!voxel_of_thread(big, half, x, sy, sz))
 	{
-		const int x = static_cast<int>(i % half);
-		const int sy = static_cast<int>((i / half) % big);
-		const int sz = static_cast<int>(i / (static_cast<int64_t>(half) * big));
-		const int y = sy <= big / 2 ? sy : sy - big;
-		const int z = sz <= big / 2 ? sz : sz - big;
-		float f[16];
-		f[0] = sqrtf(static_cast<float>(x * x + y * y + z * z)) / (0.5f * static_cast<float>(big));
-		f[1] = log1pf(fmaxf(den[i], 0.0f));
-		int bin = 0;
-		for (int oz = -1; oz <= 1; oz += 2)
+		return;
+	}
+	const int64_t i = (static_cast<int64_t>(sz) * big + sy) * half + x;
+	const int y = sy <= big / 2 ? sy : sy - big;
+	const int z = sz <= big / 2 ? sz : sz - big;
+	float nb[27];
+	for (int dz = -1; dz <= 1; ++dz)
+	{
+		for (int dy = -1; dy <= 1; ++dy)
 		{
-			for (int oy = -1; oy <= 1; oy += 2)
+			for (int dx = -1; dx <= 1; ++dx)
 			{
-				for (int ox = -1; ox <= 1; ox += 2)
-				{
-					float sum = 0.0f;
-					for (int c = 1; c < 8; ++c)
-					{
-						const int dx = (c & 1) ? ox : 0;
-						const int dy = (c & 2) ? oy : 0;
-						const int dz = (c & 4) ? oz : 0;
-						sum += fmaxf(den[hermitian_index(x + dx, y + dy, z + dz, big, half)], 0.0f);
-					}
-					f[2 + bin] = log1pf(sum / 7.0f);
-					++bin;
-				}
+				nb[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1] =
+					fmaxf(den[neighbour_index(x + dx, y + dy, z + dz, big, half)], 0.0f);
 			}
 		}
-		for (int k = 10; k < 16; ++k)
+	}
+	float f[16];
+	f[0] = sqrtf(static_cast<float>(x * x + y * y + z * z)) / (0.5f * static_cast<float>(big));
+	f[1] = log1pf(fmaxf(den[i], 0.0f));
+	int bin = 0;
+	for (int oz = -1; oz <= 1; oz += 2)
+	{
+		for (int oy = -1; oy <= 1; oy += 2)
 		{
-			f[k] = 0.0f;
-		}
-		for (int k = 0; k < 16; ++k)
-		{
-			features[i * 16 + k] = __float2half(f[k]);
+			for (int ox = -1; ox <= 1; ox += 2)
+			{
+				float sum = 0.0f;
+				for (int c = 1; c < 8; ++c)
+				{
+					const int dx = (c & 1) ? ox : 0;
+					const int dy = (c & 2) ? oy : 0;
+					const int dz = (c & 4) ? oz : 0;
+					sum += nb[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1];
+				}
+				f[2 + bin] = log1pf(sum / 7.0f);
+				++bin;
+			}
 		}
 	}
+	for (int k = 10; k < 16; ++k)
+	{
+		f[k] = 0.0f;
+	}
+	unsigned packed[8];
+	for (int k = 0; k < 8; ++k)
+	{
+		packed[k] = static_cast<unsigned>(__half_as_ushort(__float2half(f[2 * k]))) |
+			(static_cast<unsigned>(__half_as_ushort(__float2half(f[2 * k + 1]))) << 16);
+	}
+	uint4 *out = reinterpret_cast<uint4 *>(features + i * 16);
+	out[0] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+	out[1] = make_uint4(packed[4], packed[5], packed[6], packed[7]);
 }
 
 // ---------------------------------------------------------------------------
@@ -605,6 +637,13 @@ __global__ void adam_kernel(float *params, _Float16 *params_h, float *grads, flo
 	params_h[i] = static_cast<_Float16>(p);
 }
 
+// The (ceil(N (N/2 + 1) / 256), N) grid of voxel_of_thread.
+dim3 volume_grid(int big)
+{
+	const int64_t plane = static_cast<int64_t>(big) * (big / 2 + 1);
+	return dim3(static_cast<unsigned>((plane + 255) / 256), static_cast<unsigned>(big));
+}
+
 unsigned blocks_of(int64_t count, int per)
 {
 	return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((count + per - 1) / per, 16384)));
@@ -628,7 +667,7 @@ void launch_compensation(const float *den, float *w, float *scratch, int big, in
 	fill_kernel<<<blocks_of(count, 256), 256, 0, stream>>>(a, count, 1.0f);
 	for (int p = 0; p < passes; ++p)
 	{
-		compensation_pass_kernel<<<blocks_of(count, 256), 256, 0, stream>>>(den, a, b, big, floor_value);
+		compensation_pass_kernel<<<volume_grid(big), 256, 0, stream>>>(den, a, b, big, floor_value);
 		float *t = a;
 		a = b;
 		b = t;
@@ -642,8 +681,7 @@ void launch_compensation(const float *den, float *w, float *scratch, int big, in
 
 void launch_features(const float *den, void *features, int big, hipStream_t stream)
 {
-	const int64_t count = static_cast<int64_t>(big) * big * (big / 2 + 1);
-	features_kernel<<<blocks_of(count, 256), 256, 0, stream>>>(den, static_cast<__half *>(features), big);
+	features_kernel<<<volume_grid(big), 256, 0, stream>>>(den, static_cast<__half *>(features), big);
 	REXLIB_HIP_CHECK(hipGetLastError());
 }
 
