@@ -179,6 +179,7 @@ __device__ inline void wave_order()
 }
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int hidden = 64;
@@ -203,6 +204,21 @@ constexpr int parameter_count = off_b4 + 16;
 __device__ inline float4_t mfma16(half4_t a, half4_t b, float4_t c)
 {
 	return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// D (16x16, float) += A (16x32, fp16) B (32x16, fp16), gfx950's double-depth
+// form: twice the products of mfma16 per instruction. Lane l holds eight
+// values of A's row l % 16 and eight of B's column l % 16, the same eight
+// depth positions in both; D is laid out as for mfma16.
+__device__ inline float4_t mfma32(half8_t a, half8_t b, float4_t c)
+{
+	return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// Two half4 fragments as one half8.
+__device__ inline half8_t join(half4_t lo, half4_t hi)
+{
+	return half8_t{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
 }
 
 // Load the B operand tile (rows k0.., columns n0..) of a row-major K x N
@@ -322,10 +338,16 @@ constexpr int mlp_waves = 4;
 // computes Y^T = W^T X^T on a tile of 16 samples, the 16 columns of the B
 // operand. The weights are the A operands and stay in registers for the
 // whole kernel. An MFMA's output fragment (lane l: rows 4 (l / 16) + j,
-// column l % 16) is laid out as the B operand of the next layer's k-tile, so
-// after bias, ReLU and the float16 rounding the activations never leave
-// registers. Lane l of A holds W^T[m0 + l % 16][k0 + 4 (l / 16) + j] =
-// W[k0 + 4 (l / 16) + j][m0 + l % 16].
+// column l % 16) is laid out as a B operand, so after bias, ReLU and the
+// float16 rounding the activations never leave registers.
+// - The first layer (depth 16) takes mfma16: lane l of A holds
+//   W^T[m0 + l % 16][4 (l / 16) + j] = W[4 (l / 16) + j][m0 + l % 16].
+// - The depth-64 layers take mfma32 over two chunks of 32 hidden units. A
+//   sum is the same in any order of its terms, so chunk t gives lane group
+//   g = l / 16 the units 32 t + 4 g + j (j < 4) and 32 t + 16 + 4 g + j - 4
+//   (j >= 4): its B operand is the two output fragments of m-tiles 2t and
+//   2t + 1 side by side, and its A operand holds the weights of the same
+//   units.
 __global__ __launch_bounds__(256) void mlp_forward_kernel(const __half *features, const _Float16 *params_h,
 	const float *params_f, float *out, int64_t count)
 {
@@ -344,9 +366,9 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const __half *features
 	const int g = lane >> 4;
 	const int c = lane & 15;
 	half4_t w1[4];
-	half4_t w2[4][4];
-	half4_t w3[4][4];
-	half4_t w4[4];
+	half8_t w2[4][2];
+	half8_t w3[4][2];
+	half8_t w4[2];
 #pragma unroll
 	for (int j = 0; j < 4; ++j)
 	{
@@ -354,17 +376,23 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const __half *features
 		for (int mt = 0; mt < 4; ++mt)
 		{
 			w1[mt][j] = params_h[off_w1 + (4 * g + j) * hidden + mt * 16 + c];
-#pragma unroll
-			for (int kt = 0; kt < 4; ++kt)
-			{
-				w2[mt][kt][j] = params_h[off_w2 + (kt * 16 + 4 * g + j) * hidden + mt * 16 + c];
-				w3[mt][kt][j] = params_h[off_w3 + (kt * 16 + 4 * g + j) * hidden + mt * 16 + c];
-			}
 		}
+	}
 #pragma unroll
-		for (int kt = 0; kt < 4; ++kt)
+	for (int j = 0; j < 8; ++j)
+	{
+#pragma unroll
+		for (int t = 0; t < 2; ++t)
 		{
-			w4[kt][j] = params_h[off_w4 + (kt * 16 + 4 * g + j) * 16 + c];
+			// Hidden unit of depth position (g, j) in chunk t.
+			const int unit = 32 * t + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
+#pragma unroll
+			for (int mt = 0; mt < 4; ++mt)
+			{
+				w2[mt][t][j] = params_h[off_w2 + unit * hidden + mt * 16 + c];
+				w3[mt][t][j] = params_h[off_w3 + unit * hidden + mt * 16 + c];
+			}
+			w4[t][j] = params_h[off_w4 + unit * 16 + c];
 		}
 	}
 	const float b4 = params_f[off_b4];
@@ -398,9 +426,9 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const __half *features
 		{
 			float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-			for (int kt = 0; kt < 4; ++kt)
+			for (int t = 0; t < 2; ++t)
 			{
-				acc = mfma16(w2[mt][kt], h1[kt], acc);
+				acc = mfma32(w2[mt][t], join(h1[2 * t], h1[2 * t + 1]), acc);
 			}
 			const float4_t b = bias[1][mt][g];
 #pragma unroll
@@ -414,9 +442,9 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const __half *features
 		{
 			float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-			for (int kt = 0; kt < 4; ++kt)
+			for (int t = 0; t < 2; ++t)
 			{
-				acc = mfma16(w3[mt][kt], h2[kt], acc);
+				acc = mfma32(w3[mt][t], join(h2[2 * t], h2[2 * t + 1]), acc);
 			}
 			const float4_t b = bias[2][mt][g];
 #pragma unroll
@@ -427,9 +455,9 @@ __global__ __launch_bounds__(256) void mlp_forward_kernel(const __half *features
 		}
 		float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-		for (int kt = 0; kt < 4; ++kt)
+		for (int t = 0; t < 2; ++t)
 		{
-			acc = mfma16(w4[kt], h1[kt], acc);
+			acc = mfma32(w4[t], join(h1[2 * t], h1[2 * t + 1]), acc);
 		}
 		// Output row 0 (the one used column of W4) sits in lanes 0..15.
 		if (g == 0 && row < count)
