@@ -40,6 +40,16 @@ def test_rfft2_odd_batch_and_strides():
     assert rel(out.cpu().numpy(), np.fft.rfft2(x.transpose(0, 2, 1).astype(np.float64))) < 2e-6
 
 
+def test_rfft2_strided_rows_at_256():
+    # Rows with an element stride of 256 take the four-step row kernel's
+    # element-by-element loads rather than its 16-byte ones.
+    x = np.random.default_rng(2).normal(size=(3, 256, 256)).astype(np.float32)
+    t = torch.from_numpy(x).cuda().transpose(1, 2)
+    out = rexlib_amd.rfft2(t)
+    torch.cuda.synchronize()
+    assert rel(out.cpu().numpy(), np.fft.rfft2(x.transpose(0, 2, 1).astype(np.float64))) < 2e-6
+
+
 @pytest.mark.parametrize("n", [8, 12, 64, 128])
 def test_irfft2_inverts_rfft2(n):
     x = np.random.default_rng(n).normal(size=(4, n, n)).astype(np.float32)

@@ -67,12 +67,14 @@ def test_fused_images_equal_rfft2_then_insertion():
     assert oracle.rel_l2(den.cpu().numpy(), den2.cpu().numpy()) < 1e-6
 
 
-@pytest.mark.parametrize("schedule", ["auto", "scatter"])
-def test_fused_images_at_256_equal_rfft2_then_insertion(schedule, monkeypatch):
+@pytest.mark.parametrize("schedule,m", [("auto", 128), ("scatter", 128), ("auto", 130)])
+def test_fused_images_at_256_equal_rfft2_then_insertion(schedule, m, monkeypatch):
     # 256-point slices take the column transforms fused with the sample
-    # preparation (brick schedule) or run them on their own (scatter).
+    # preparation (brick schedule) or run them on their own (scatter). With
+    # m = 130 the image starts 63 columns in, off the 4-element alignment of
+    # the row kernel's 16-byte loads, so it loads element by element.
     monkeypatch.setenv("REXLIB_HIP_INSERTION_SCHEDULE", schedule)
-    stack = make_stack(count=40, m=128, n=256)
+    stack = make_stack(count=40, m=m, n=256)
     n, big = 256, 64
     num, den = gpu_insert(stack, "f32", n, big)
     images = torch.from_numpy(padded(stack.images.cpu().numpy(), n).astype(np.float32)).cuda()
