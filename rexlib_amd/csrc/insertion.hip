@@ -726,44 +726,13 @@ struct slice_chunk
 	int64_t count;
 };
 
-template <int VARIANT>
-__global__ __launch_bounds__(256) void prepare_kernel(insertion_params p, slice_chunk chunk, prepared_sample *prepared)
-{
-	// A slice holds n (n/2 + 1) < 2^31 elements: 32-bit indices (a 64-bit
-	// division per element would cost more than the sample itself).
-	const int elements32 = p.n * p.h;
-	const int64_t elements = elements32;
-	const int idx = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-	if (idx >= elements32)
-	{
-		return;
-	}
-	const int row = static_cast<int>(static_cast<unsigned>(idx) / static_cast<unsigned>(p.h));
-	const int column = idx - row * p.h;
-	const int64_t slices = p.map.kept_count * chunk.count;
-	for (int64_t j = blockIdx.y; j < slices; j += gridDim.y)
-	{
-		const int64_t kept = j / chunk.count;
-		const int64_t s = kept * p.map.reduced_count + chunk.begin + j % chunk.count;
-		const slice_entry &e = p.entries[s];
-		float2 value = make_float2(0.0f, 0.0f);
-		float weight = 0.0f;
-		if (e.last_shell < 0 || !prepare_sample<VARIANT>(p, e, s, row, column, value, weight))
-		{
-			value = make_float2(0.0f, 0.0f);
-			weight = 0.0f;
-		}
-		prepared[j * elements + idx] = prepared_sample{value.x, value.y, weight};
-	}
-}
-
-// prepare_value without branches, for the fused column pass: every load is
-// in bounds whatever the sample (the radial index is clamped to the last
-// shell, the phase tables cover every row and column), and the sample is
-// zeroed afterwards where prepare_value rejects it. The loads of several rows
-// can then be in flight together. The arithmetic is prepare_value's, in the
-// same order.
-// `radial` is the slice's radial weights, `y_table` its row phases.
+// prepare_value without branches, for prepare_kernel and the fused column
+// passes: every load is in bounds whatever the sample (the radial index is
+// clamped to the last shell, the phase tables cover every row and column),
+// and the sample is zeroed afterwards where prepare_value rejects it, so the
+// loads of several elements can be in flight together. The arithmetic is
+// prepare_value's, in the same order. `radial` is the slice's radial
+// weights, `y_table` its row phases.
 template <int VARIANT>
 __device__ inline prepared_sample prepare_unbranched(const insertion_params &p, const slice_entry &e,
 	const float *radial, const float2 *y_table, float2 x_phase, int row, int column, float2 x)
@@ -797,6 +766,35 @@ __device__ inline prepared_sample prepare_unbranched(const insertion_params &p, 
 	const bool inserted = !nyquist && shell <= e.last_shell && r != 0.0f;
 	return inserted ? prepared_sample{v.x * scale, v.y * scale, r * denominator_weight} :
 		prepared_sample{0.0f, 0.0f, 0.0f};
+}
+
+template <int VARIANT>
+__global__ __launch_bounds__(256) void prepare_kernel(insertion_params p, slice_chunk chunk, prepared_sample *prepared)
+{
+	// A slice holds n (n/2 + 1) < 2^31 elements: 32-bit indices (a 64-bit
+	// division per element would cost more than the sample itself).
+	const int elements32 = p.n * p.h;
+	const int64_t elements = elements32;
+	const int idx = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+	if (idx >= elements32)
+	{
+		return;
+	}
+	const int row = static_cast<int>(static_cast<unsigned>(idx) / static_cast<unsigned>(p.h));
+	const int column = idx - row * p.h;
+	const int64_t slices = p.map.kept_count * chunk.count;
+	for (int64_t j = blockIdx.y; j < slices; j += gridDim.y)
+	{
+		const int64_t kept = j / chunk.count;
+		const int64_t s = kept * p.map.reduced_count + chunk.begin + j % chunk.count;
+		const slice_entry &e = p.entries[s];
+		// Branch-free, as in the fused column pass: every element is loaded
+		// and the rejected ones zeroed afterwards.
+		const float2 *table = p.tables + s * static_cast<int64_t>(p.h + p.n);
+		const float2 x = p.slices[e.slice_offset + row * p.slice_s0 + column * p.slice_s1];
+		prepared[j * elements + idx] = prepare_unbranched<VARIANT>(p, e, p.radial + e.radial_offset, table + p.h,
+			table[column], row, column, x);
+	}
 }
 
 // The column transforms of 256-point slices fused with the preparation.
