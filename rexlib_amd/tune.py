@@ -23,7 +23,7 @@ TABLE = os.path.join(HERE, "tuning.json")
 
 EXTENTS_Y = (8, 16)
 EXTENTS_Z = (6, 8, 16)
-SHARES = (64, 128, 256, 512, 1024)
+SHARES = (64, 96, 128, 192, 256, 512, 1024)
 
 
 def load_table(path: str = TABLE) -> dict:
