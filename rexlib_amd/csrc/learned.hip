@@ -127,9 +127,12 @@ __global__ __launch_bounds__(256) void features_kernel(const float *den, __half 
 			}
 		}
 	}
+	// log(1 + v) by the hardware logarithm: the features are stored as
+	// float16, whose rounding (about 5e-4 relative) is far above its error.
+	auto log1 = [](float v) { return __logf(1.0f + v); };
 	float f[16];
 	f[0] = sqrtf(static_cast<float>(x * x + y * y + z * z)) / (0.5f * static_cast<float>(big));
-	f[1] = log1pf(fmaxf(den[i], 0.0f));
+	f[1] = log1(fmaxf(den[i], 0.0f));
 	int bin = 0;
 	for (int oz = -1; oz <= 1; oz += 2)
 	{
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(256) void features_kernel(const float *den, __half 
 					const int dz = (c & 4) ? oz : 0;
 					sum += nb[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1];
 				}
-				f[2 + bin] = log1pf(sum / 7.0f);
+				f[2 + bin] = log1(sum / 7.0f);
 				++bin;
 			}
 		}
