@@ -57,7 +57,7 @@ def traffic_per_launch(args, tuned):
     if record.get("n") != args.n or record.get("N") != args.big or record.get("launch") != args.launch:
         return None
     brick = record.get("brick", {})
-    if tuned is None or (brick.get("extent_y"), brick.get("extent_z"), brick.get("share")) != tuple(tuned):
+    if tuned is None or (tuple(brick.get("shift", ())), brick.get("records_per_task")) != (tuple(tuned[0]), tuned[1]):
         return None  # measured for another brick shape
     return record["bytes_per_launch"]
 
@@ -74,9 +74,42 @@ def make_workload(args, torch):
     return stack
 
 
+def host_cpu():
+    """(logical cores, model name) of the host, from os.cpu_count() and lscpu."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return os.cpu_count() or 1, model
+
+
+def cpu_share():
+    """Host threads for the CPU baseline: the cores this process may run on,
+    at most OMP_NUM_THREADS (the GPU boxes allot 16 host cores per GPU and
+    set it so)."""
+    try:
+        available = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        available = os.cpu_count() or 1
+    limit = os.environ.get("OMP_NUM_THREADS")
+    if limit and limit.isdigit() and int(limit) > 0:
+        available = min(available, int(limit))
+    return max(1, available)
+
+
 def cpu_baseline(args, stack, threads):
-    """The reference's insertion kernels (oracle/_ref) plus pocketfft rfft2 on
-    the host, over a bounded sample of the same images, timed on `threads` cores."""
+    """The reference's CPU path on the host cores: pocketfft rfft2 (scipy.fft)
+    and the reference's insertion kernels (oracle/_ref), over a bounded sample
+    of the same images; then the weight quotient and the 3-D inverse FFT of
+    the accumulators (numpy and scipy.fft), timed once, so that the
+    end-to-end figure is the whole reconstruction of `args.images` images at
+    the sampled insertion rate."""
     import oracle
     import scipy.fft
     if not oracle.ref_available():
@@ -105,14 +138,28 @@ def cpu_baseline(args, stack, threads):
         elapsed += time.perf_counter() - t0
         done += chunk
         start += chunk
+    t0 = time.perf_counter()
+    quotient = np.zeros_like(acc)
+    mask = den > 1e-3
+    quotient[mask] = acc[mask] / den[mask]
+    scipy.fft.irfftn(quotient, s=(big, big, big), workers=threads)
+    finish = time.perf_counter() - t0
+    rate = done / elapsed
+    cores, model = host_cpu()
     return {
-        "value": done / elapsed,
+        "value": rate,
         "unit": "images/s",
         "cores": threads,
         "kind": "reference",
+        "host_cores": cores,
+        "host_cpu": model,
+        "thread_limit": "the cores this process may use, at most OMP_NUM_THREADS (16 host cores per GPU on the box)",
+        "end_to_end_images_per_s": args.images / (args.images / rate + finish),
+        "quotient_irfft3_s": finish,
         "sample": (f"{done} images of {n}^2 into one {big}^3 grid (rfft2 by scipy.fft/pocketfft, "
                    f"CTF trilinear insertion by the reference's row kernel, tile-owned over "
-                   f"{threads} threads), {elapsed:.1f} s"),
+                   f"{threads} threads), {elapsed:.1f} s; then quotient + irfft3 of the {big}^3 grid "
+                   f"({finish:.2f} s, scipy.fft, {threads} workers)"),
     }
 
 
@@ -292,9 +339,11 @@ def main():
     window = em.GriddingWindow.linear()
     optics = em.CtfOptics(300.0, 2.7, 0.1)
     launches = [(s, min(s + L, args.images)) for s in range(0, args.images, L)]
-    # The autotuner's brick shape and share for this box size (rexlib_amd/tuning.json).
-    tuned = tune.apply(n, big)
-    log(f"brick tuning (extent_y, extent_z, share): {tuned}")
+    # The autotuner's table (rexlib_amd/tuning.json); the library picks the
+    # entry of this box size.
+    tune.apply_all()
+    tuned = tune.get_tuning(big)
+    log(f"brick tuning (log2 brick extents, records per task): {tuned}")
 
     def step():
         num.zero_()
@@ -330,6 +379,7 @@ def main():
     elapsed = time.perf_counter() - t0
     brick_ms, brick_launches = _native.kernel_timing(_native.KERNEL_BRICK)
     prepare_ms, prepare_launches = _native.kernel_timing(_native.KERNEL_PREPARE)
+    count_ms, count_launches = _native.kernel_timing(_native.KERNEL_BIN_COUNT)
     _native.set_kernel_timing(False)
     ms_per_step = 1e3 * elapsed / args.steps
     value = args.images * args.steps / elapsed
@@ -347,7 +397,7 @@ def main():
     achieved = b_img * images_timed / (brick_ms * 1e-3) / 1e9 if brick_launches else None
     roofline = {
         "bound": "hbm",
-        "kernel": "brick_kernel (LDS-owned brick insertion, CTF + trilinear)",
+        "kernel": "bin_brick_kernel (binned LDS brick insertion, CTF + trilinear)",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -358,6 +408,7 @@ def main():
         "images_per_launch": images_per_launch,
         "bytes_per_image": b_img,
         "column_prepare_kernel_ms": prepare_ms / prepare_launches if prepare_launches else None,
+        "bin_count_and_plan_ms": count_ms / count_launches if count_launches else None,
         "pipeline_frac": value * b_img / (HBM_PEAK_GBS * 1e9),
     }
 
@@ -376,7 +427,7 @@ def main():
     parity = None if args.no_parity else parity_sample(args, stack, torch, em, rexlib_amd)
     if parity:
         log(f"parity: {parity}")
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_share()
     baseline = None if args.no_cpu_baseline else cpu_baseline(args, stack, threads)
     if baseline:
         log(f"cpu baseline: {baseline}")
@@ -398,7 +449,7 @@ def main():
             "workload": (f"plain FP32: {args.images} images {n}^2 -> {big}^3, launches of {L}, fused rfft2 + "
                          "CTF trilinear insertion, then quotient, irfft3, gridding correction"),
             "images": args.images, "n": n, "N": big, "launch": L, "window": "linear", "variant": "ctf",
-            "brick": {"extent_x": 16, "extent_y": tuned[0], "extent_z": tuned[1], "share": tuned[2]} if tuned else None,
+            "brick": {"shift": list(tuned[0]), "records_per_task": tuned[1]} if tuned else None,
         },
         "roofline": roofline,
         "cpu_baseline": baseline,

@@ -93,15 +93,22 @@ void rexlib_hip_version(int32_t *major, int32_t *minor, int32_t *patch);
  * (plugins/cuda/src/hardware/cuda_device_backend.cpp:37-42). */
 int32_t rexlib_hip_device_count(void);
 
-/* Tuning of the LDS brick schedule of the insertions, process-wide: brick
- * extents along y (8 or 16) and z (6, 8 or 16; 6 with 8 along y), 16 along
- * x, and the expected plane hits per task that splits busy bricks (>= 16).
- * 0 keeps the default (16 x 8 x 6, 256 hits).
+/* Tuning of the brick schedule of the insertions, per volume extent N
+ * (extent 0: every extent without an entry of its own): the brick of
+ * first-tap cells a wavefront owns in LDS, as log2 of its extents along x, y
+ * and z -- (3, 3, 3), (4, 3, 3), (3, 3, 2) or (4, 3, 2) -- and the records
+ * (sample copies) per task, in [256, 2^24], that splits busy bricks. Each
+ * call reads its extent's entry once; without entries the default is
+ * (3, 3, 3) and 8192. Returns REXLIB_HIP_INVALID_ARGUMENT for other values.
  * No reference counterpart (the reference's CPU tuning is
  * slice_insertion_tuning, src/backends/cpu/builders/em/reconstruction/slice_insertion_tuning.cpp:10-13);
  * set by the autotuner (rexlib_amd/tune.py). */
-void rexlib_hip_set_insertion_tuning(int32_t extent_y, int32_t extent_z, int32_t share);
-void rexlib_hip_get_insertion_tuning(int32_t *extent_y, int32_t *extent_z, int32_t *share);
+int rexlib_hip_set_insertion_tuning(int32_t extent, int32_t shift_x, int32_t shift_y, int32_t shift_z,
+	int32_t records_per_task);
+int rexlib_hip_get_insertion_tuning(int32_t extent, int32_t *shift_x, int32_t *shift_y, int32_t *shift_z,
+	int32_t *records_per_task);
+/* Drops every entry (the defaults apply again). */
+void rexlib_hip_clear_insertion_tuning(void);
 
 /*
  * Event timing of single kernels, for measurement. When switched on, the
@@ -114,10 +121,11 @@ void rexlib_hip_get_insertion_tuning(int32_t *extent_y, int32_t *extent_z, int32
 typedef enum rexlib_hip_kernel
 {
     REXLIB_HIP_KERNEL_BRICK = 0,   /* brick insertion kernel */
-    REXLIB_HIP_KERNEL_PREPARE = 1, /* per-sample preparation for the brick kernel; for 256-point
-                                      images through rexlib_hip_insert_ctf_images, the column
-                                      transforms fused with it */
-    REXLIB_HIP_KERNEL_SCATTER = 2  /* atomic scatter insertion kernel */
+    REXLIB_HIP_KERNEL_PREPARE = 1, /* per-sample preparation writing the brick schedule's bins;
+                                      for images through rexlib_hip_insert_ctf_images, the
+                                      column transforms fused with it */
+    REXLIB_HIP_KERNEL_SCATTER = 2, /* atomic scatter insertion kernel */
+    REXLIB_HIP_KERNEL_BIN_COUNT = 3 /* counting pass and bin plan of the brick schedule */
 } rexlib_hip_kernel;
 
 int rexlib_hip_set_kernel_timing(int32_t enable);
@@ -248,13 +256,16 @@ int rexlib_hip_irfft2(const rexlib_hip_array *spectra, const rexlib_hip_array *i
 /* ------------------------------------------------------------------------- */
 
 /* out = numerator / denominator where denominator > threshold, else 0.
- * Contiguous (N, N, N/2+1); out may alias numerator. */
+ * Contiguous (..., N, N, N/2+1) of one shape; out may alias numerator. */
 int rexlib_hip_normalize(const rexlib_hip_array *numerator, const rexlib_hip_array *denominator,
 	double threshold, const rexlib_hip_array *out, void *stream);
 
-/* Divide a real volume (N, N, N) with its origin at N/2 by the separable
+/* Divide a real volume (..., N, N, N) with its origin at N/2 by the separable
  * transform of the window: sinc (box), sinc^2 (triangle), or the Kaiser-Bessel
- * closed form, each normalised to 1 at the origin. out may alias volume. */
+ * closed form, each normalised to 1 at the origin. out is (..., M, M, M) with
+ * M <= N: for M < N it receives the central M^3 (starting at N/2 - M/2, the
+ * crop of a reconstruction inserted with padding), else the whole volume, and
+ * it may then alias volume. Contiguous operands. */
 int rexlib_hip_gridding_correct(const rexlib_hip_array *volume, const rexlib_hip_window *window,
 	const rexlib_hip_array *out, void *stream);
 

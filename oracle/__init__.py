@@ -333,11 +333,17 @@ def window_transform(window: Window, extent: int):
     return kb(z2) / kb(np.array([b * b]))[0]
 
 
-def gridding_correct(volume, window: Window):
-    """Divide a real-space volume (origin at C) by the window's transform per axis."""
+def gridding_correct(volume, window: Window, crop=None):
+    """Divide a real-space volume (origin at C = N // 2) by the window's
+    transform per axis; with ``crop`` = M, keep only the central M^3, from
+    C - M // 2 (the crop of a reconstruction inserted with padding)."""
     extent = volume.shape[-1]
     c = window_transform(window, extent)
-    return volume / (c[:, None, None] * c[None, :, None] * c[None, None, :])
+    out = volume / (c[:, None, None] * c[None, :, None] * c[None, None, :])
+    if crop is None or crop == extent:
+        return out
+    o = extent // 2 - crop // 2
+    return out[..., o:o + crop, o:o + crop, o:o + crop]
 
 
 def shell_index(extent):
@@ -396,16 +402,32 @@ def density_compensation(den, passes=10, floor=1e-6):
     d_half = np.asarray(den, dtype=np.float64)
     d = full_from_half(d_half, extent)
     w_half = np.ones_like(d_half)
-    t = {-1: 1 / 6, 0: 2 / 3, 1: 1 / 6}
     for _ in range(passes):
-        prod = full_from_half(w_half, extent) * d
-        c = np.zeros_like(prod)
-        for dz in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dx in (-1, 0, 1):
-                    c += t[dz] * t[dy] * t[dx] * np.roll(prod, (-dz, -dy, -dx), axis=(0, 1, 2))
+        c = convolve_k3(full_from_half(w_half, extent) * d)
         w_half = np.where(d_half > 0, w_half / np.maximum(c[:, :, :h], floor), 0.0)
     return w_half
+
+
+K3 = {-1: 1 / 6, 0: 2 / 3, 1: 1 / 6}
+
+
+def convolve_k3_direct(field):
+    """The periodic 27-point sum of K3(dz) K3(dy) K3(dx) field[g + d]."""
+    c = np.zeros_like(field)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                c += K3[dz] * K3[dy] * K3[dx] * np.roll(field, (-dz, -dy, -dx), axis=(0, 1, 2))
+    return c
+
+
+def convolve_k3(field):
+    """convolve_k3_direct by its three axes in turn (the kernel is a product
+    of one 1-D kernel per axis), 9 shifted sums instead of 27."""
+    c = field
+    for axis in (0, 1, 2):
+        c = K3[0] * c + K3[1] * (np.roll(c, -1, axis=axis) + np.roll(c, 1, axis=axis))
+    return c
 
 
 def compensation_features(den):

@@ -19,6 +19,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <stdlib.h>
 
 #define ORACLE_PI 3.14159265358979323846
 

@@ -57,6 +57,7 @@ EXPORTS = [
     "rexlib_hip_device_count",
     "rexlib_hip_set_insertion_tuning",
     "rexlib_hip_get_insertion_tuning",
+    "rexlib_hip_clear_insertion_tuning",
     "rexlib_hip_insert_slices",
     "rexlib_hip_insert_weighted_slices",
     "rexlib_hip_insert_ctf_slices",
@@ -99,10 +100,12 @@ def library() -> ctypes.CDLL:
     lib.rexlib_hip_version.restype = None
     lib.rexlib_hip_device_count.restype = ctypes.c_int32
     lib.rexlib_hip_device_count.argtypes = []
-    lib.rexlib_hip_set_insertion_tuning.argtypes = [ctypes.c_int32] * 3
-    lib.rexlib_hip_set_insertion_tuning.restype = None
-    lib.rexlib_hip_get_insertion_tuning.argtypes = [P(ctypes.c_int32)] * 3
-    lib.rexlib_hip_get_insertion_tuning.restype = None
+    lib.rexlib_hip_set_insertion_tuning.argtypes = [ctypes.c_int32] * 5
+    lib.rexlib_hip_set_insertion_tuning.restype = ctypes.c_int
+    lib.rexlib_hip_get_insertion_tuning.argtypes = [ctypes.c_int32] + [P(ctypes.c_int32)] * 4
+    lib.rexlib_hip_get_insertion_tuning.restype = ctypes.c_int
+    lib.rexlib_hip_clear_insertion_tuning.argtypes = []
+    lib.rexlib_hip_clear_insertion_tuning.restype = None
     lib.rexlib_hip_insert_slices.argtypes = [A, A, A, A, A, W, A, A, vp]
     lib.rexlib_hip_insert_weighted_slices.argtypes = [A, A, A, A, A, A, W, A, A, vp]
     lib.rexlib_hip_insert_ctf_slices.argtypes = [A, A, A, A, A, A, W, O, ctypes.c_double, A, A, vp]
@@ -162,7 +165,7 @@ def check(status: int) -> None:
     raise RuntimeError(message)
 
 
-KERNEL_BRICK, KERNEL_PREPARE, KERNEL_SCATTER = 0, 1, 2
+KERNEL_BRICK, KERNEL_PREPARE, KERNEL_SCATTER, KERNEL_BIN_COUNT = 0, 1, 2, 3
 
 
 def set_kernel_timing(enable: bool) -> None:

@@ -52,11 +52,10 @@ WINDOW = em.GriddingWindow.linear()
 OPTICS = em.CtfOptics(300.0, 2.7, 0.1)
 
 
-def run_one(r: Reconstruction, context: em.ExecutionContext | None = None, tuning: bool = True):
-    """The whole path of one reconstruction, enqueued on the context's stream."""
-    n = r.extent
-    if tuning:
-        tune.apply(n, n)
+def run_one(r: Reconstruction, context: em.ExecutionContext | None = None):
+    """The whole path of one reconstruction, enqueued on the context's stream.
+    The library picks the brick tuning of the reconstruction's extent itself
+    (``tune.apply_all`` hands it the table once)."""
     r.numerator.zero_()
     r.denominator.zero_()
     count = r.images.shape[0]
@@ -86,6 +85,8 @@ class BatchedReconstruction:
     """All reconstructions captured in one HIP graph over `streams` streams."""
 
     def __init__(self, recons, streams: int = 4, tuning: bool = True):
+        if tuning:
+            tune.apply_all()
         self.recons = recons
         self.tuning = tuning
         for r in recons:
@@ -101,7 +102,7 @@ class BatchedReconstruction:
             ctx = em.ExecutionContext(stream=stream)
             with torch.cuda.stream(stream):
                 for i in group:
-                    run_one(self.recons[i], ctx, self.tuning)
+                    run_one(self.recons[i], ctx)
         for stream in self.streams:
             origin.wait_stream(stream)
 
@@ -123,7 +124,9 @@ class BatchedReconstruction:
 
 def run_sequential(recons, tuning: bool = True):
     """The same reconstructions one after another on the current stream."""
+    if tuning:
+        tune.apply_all()
     for r in recons:
         if r.numerator is None:
             r.allocate()
-        run_one(r, None, tuning)
+        run_one(r, None)

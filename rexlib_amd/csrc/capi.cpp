@@ -7,6 +7,7 @@
 #include <sstream>
 #include <string>
 #include <tuple>
+#include <vector>
 
 namespace rexlib_hip
 {
@@ -24,6 +25,7 @@ struct area_entry
 
 std::mutex area_mutex;
 std::map<std::tuple<int, hipStream_t, int>, area_entry> areas;
+std::vector<void *> retired_areas;
 
 } // anonymous namespace
 
@@ -40,16 +42,27 @@ void *workspace_area_bytes(int area, std::size_t bytes, hipStream_t stream)
 	auto &entry = areas[std::make_tuple(device, stream, area)];
 	if (entry.bytes < bytes)
 	{
-		if (entry.pointer != nullptr)
+		// A graph captured on this stream holds the current area's address:
+		// growing while a capture is under way would leave it pointing at a
+		// smaller area, so the areas are sized by an eager call first.
+		hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+		REXLIB_HIP_CHECK(hipStreamIsCapturing(stream, &capture));
+		if (capture != hipStreamCaptureStatusNone)
 		{
-			// Work already enqueued may still read the old area.
-			REXLIB_HIP_CHECK(hipStreamSynchronize(stream));
-			REXLIB_HIP_CHECK(hipFree(entry.pointer));
-			entry.pointer = nullptr;
-			entry.bytes = 0;
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "workspace: an insertion captured into a graph needs more workspace "
+				"than an eager call of the same shape has sized on this stream; run it once eagerly before capture");
 		}
 		const std::size_t grown = std::max(bytes, entry.bytes + entry.bytes / 2);
-		REXLIB_HIP_CHECK(hipMalloc(&entry.pointer, grown));
+		void *fresh = nullptr;
+		REXLIB_HIP_CHECK(hipMalloc(&fresh, grown));
+		if (entry.pointer != nullptr)
+		{
+			// Enqueued work and graphs captured earlier may still read the old
+			// area, so it is kept until the process ends (the areas grow
+			// geometrically, so the retired ones hold at most twice the last).
+			retired_areas.push_back(entry.pointer);
+		}
+		entry.pointer = fresh;
 		entry.bytes = grown;
 	}
 	return entry.pointer;
@@ -237,6 +250,7 @@ struct insertion_request
 	int schedule;
 	hipStream_t stream;
 	const pending_columns *columns = nullptr;
+	workspace_source *workspace = nullptr; // null: the per-stream areas
 };
 
 // The shared body of the three insertions.
@@ -378,8 +392,10 @@ void insert(const insertion_request &r)
 	const int64_t num_in_strides[3] = {core_stride(num, 3), core_stride(num, 2), core_stride(num, 1)};
 	const rexlib_hip_array &den = *ops[slot_denominator];
 	const int64_t den_in_strides[3] = {core_stride(den, 3), core_stride(den, 2), core_stride(den, 1)};
+	workspace_source areas;
 	run_insertion(p, r.schedule, static_cast<const float2 *>(num.data), num_in_strides,
-		static_cast<const float *>(den.data), den_in_strides, in_place, w.shape, r.columns, r.stream);
+		static_cast<const float *>(den.data), den_in_strides, in_place, w.shape, r.columns,
+		r.workspace != nullptr ? *r.workspace : areas, r.stream);
 }
 
 int insertion_schedule_from_env()
@@ -474,27 +490,33 @@ void rexlib_hip_version(int32_t *major, int32_t *minor, int32_t *patch)
 	}
 }
 
-void rexlib_hip_set_insertion_tuning(int32_t extent_y, int32_t extent_z, int32_t share)
+int rexlib_hip_set_insertion_tuning(int32_t extent, int32_t shift_x, int32_t shift_y, int32_t shift_z,
+	int32_t records_per_task)
 {
-	rexlib_hip::set_brick_tuning(extent_y, extent_z, share);
+	return guarded([&] { rexlib_hip::set_insertion_tuning(extent, shift_x, shift_y, shift_z, records_per_task); });
 }
 
-void rexlib_hip_get_insertion_tuning(int32_t *extent_y, int32_t *extent_z, int32_t *share)
+int rexlib_hip_get_insertion_tuning(int32_t extent, int32_t *shift_x, int32_t *shift_y, int32_t *shift_z,
+	int32_t *records_per_task)
 {
-	int w = 0, d = 0, s = 0;
-	rexlib_hip::get_brick_tuning(&w, &d, &s);
-	if (extent_y)
-	{
-		*extent_y = w;
-	}
-	if (extent_z)
-	{
-		*extent_z = d;
-	}
-	if (share)
-	{
-		*share = s;
-	}
+	return guarded([&]
+		{
+			if (!shift_x || !shift_y || !shift_z || !records_per_task)
+			{
+				throw rexlib_hip::error(REXLIB_HIP_INVALID_ARGUMENT, "get_insertion_tuning: null output");
+			}
+			int sx = 0, sy = 0, sz = 0, r = 0;
+			rexlib_hip::get_insertion_tuning(extent, &sx, &sy, &sz, &r);
+			*shift_x = sx;
+			*shift_y = sy;
+			*shift_z = sz;
+			*records_per_task = r;
+		});
+}
+
+void rexlib_hip_clear_insertion_tuning(void)
+{
+	rexlib_hip::clear_insertion_tuning();
 }
 
 int rexlib_hip_set_kernel_timing(int32_t enable)
@@ -510,7 +532,7 @@ int rexlib_hip_kernel_timing(int32_t kernel, double *total_ms, int64_t *launches
 			{
 				throw rexlib_hip::error(REXLIB_HIP_INVALID_ARGUMENT, "kernel_timing: null output");
 			}
-			if (kernel < REXLIB_HIP_KERNEL_BRICK || kernel > REXLIB_HIP_KERNEL_SCATTER)
+			if (kernel < REXLIB_HIP_KERNEL_BRICK || kernel > REXLIB_HIP_KERNEL_BIN_COUNT)
 			{
 				throw rexlib_hip::error(REXLIB_HIP_INVALID_ARGUMENT, "kernel_timing: unknown kernel id");
 			}
@@ -569,6 +591,28 @@ int rexlib_hip_insert_ctf_slices(const rexlib_hip_array *numerator, const rexlib
 		insert(r);
 	});
 }
+
+} // extern "C"
+
+namespace rexlib_hip
+{
+
+void insert_ctf_slices_within(const rexlib_hip_array *numerator, const rexlib_hip_array *denominator,
+	const rexlib_hip_array *slices, const rexlib_hip_array *transforms, const rexlib_hip_array *ctf,
+	const rexlib_hip_array *radial_weights, const rexlib_hip_window *window, const rexlib_hip_ctf_optics *optics,
+	double pixel_size, const rexlib_hip_array *numerator_out, const rexlib_hip_array *denominator_out,
+	workspace_source &workspace, void *stream)
+{
+	insertion_request r{variant_ctf,
+		{numerator, denominator, slices, transforms, radial_weights, ctf, numerator_out, denominator_out},
+		window, optics, pixel_size, insertion_schedule_from_env(), as_stream(stream)};
+	r.workspace = &workspace;
+	insert(r);
+}
+
+} // namespace rexlib_hip
+
+extern "C" {
 
 int rexlib_hip_rfft2(const rexlib_hip_array *images, const rexlib_hip_array *spectra, void *stream)
 {
@@ -702,8 +746,18 @@ int rexlib_hip_normalize(const rexlib_hip_array *numerator, const rexlib_hip_arr
 		expect_dtype(*numerator, REXLIB_HIP_COMPLEX64, "normalize numerator");
 		expect_dtype(*denominator, REXLIB_HIP_FLOAT32, "normalize denominator");
 		expect_dtype(*out, REXLIB_HIP_COMPLEX64, "normalize output");
+		const int64_t big = core_extent(*numerator, 3);
+		expect_core(*numerator, "normalize numerator", {big, big, big / 2 + 1});
 		check_same_shape(*numerator, *denominator, "normalize denominator");
 		check_same_shape(*numerator, *out, "normalize output");
+		if (!contiguous(*numerator) || !contiguous(*denominator) || !contiguous(*out))
+		{
+			fail(REXLIB_HIP_UNSUPPORTED, "normalize: the gfx950 program takes contiguous operands");
+		}
+		if (!std::isfinite(threshold))
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "normalize: the threshold must be finite");
+		}
 		int64_t count = 1;
 		for (int i = 0; i < numerator->rank; ++i)
 		{
@@ -723,16 +777,43 @@ int rexlib_hip_gridding_correct(const rexlib_hip_array *volume, const rexlib_hip
 		check_array(out, "gridding_correct output", 3);
 		expect_dtype(*volume, REXLIB_HIP_FLOAT32, "gridding_correct input");
 		expect_dtype(*out, REXLIB_HIP_FLOAT32, "gridding_correct output");
-		check_same_shape(*volume, *out, "gridding_correct output");
 		const window_choice w = choose_window(window);
 		const int64_t big = core_extent(*volume, 1);
+		const int64_t crop = core_extent(*out, 1);
+		expect_core(*volume, "gridding_correct input", {big, big, big});
+		expect_core(*out, "gridding_correct output", {crop, crop, crop});
+		if (crop > big)
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "gridding_correct: the output extent exceeds the volume's");
+		}
+		if (volume->rank != out->rank)
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "gridding_correct: input and output batch shapes differ");
+		}
 		int64_t volumes = 1;
 		for (int i = 0; i < volume->rank - 3; ++i)
 		{
+			if (volume->extents[i] != out->extents[i])
+			{
+				fail(REXLIB_HIP_INVALID_ARGUMENT, "gridding_correct: input and output batch shapes differ");
+			}
 			volumes *= volume->extents[i];
 		}
+		if (!contiguous(*volume) || !contiguous(*out))
+		{
+			fail(REXLIB_HIP_UNSUPPORTED, "gridding_correct: the gfx950 program takes contiguous operands");
+		}
+		if (crop < big && volume->data == out->data)
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "gridding_correct: a cropped output cannot alias the input");
+		}
+		if (volumes == 0 || crop == 0)
+		{
+			return;
+		}
 		launch_gridding(static_cast<const float *>(volume->data), static_cast<float *>(out->data),
-			static_cast<int>(big), volumes, w.profile, w.half_width, w.shape, as_stream(stream));
+			static_cast<int>(big), static_cast<int>(crop), volumes, w.profile, w.half_width, w.shape,
+			as_stream(stream));
 	});
 }
 
@@ -747,6 +828,14 @@ int rexlib_hip_shell_sums(const rexlib_hip_array *a, const rexlib_hip_array *b, 
 		check_same_shape(*a, *b, "shell_sums b");
 		const int64_t big = core_extent(*a, 3);
 		expect_core(*a, "shell_sums a", {big, big, big / 2 + 1});
+		if (a->rank != 3 || !contiguous(*a) || !contiguous(*b))
+		{
+			fail(REXLIB_HIP_UNSUPPORTED, "shell_sums: one contiguous (N, N, N/2+1) grid per operand");
+		}
+		if (sums == nullptr)
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "shell_sums: null output");
+		}
 		launch_shell_sums(static_cast<const float2 *>(a->data), static_cast<const float2 *>(b->data), sums,
 			static_cast<int>(big), as_stream(stream));
 	});

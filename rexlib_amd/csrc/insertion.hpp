@@ -36,6 +36,14 @@ struct alignas(16) slice_entry
 	int64_t kept;
 };
 
+// The range of slices one launch covers, per kept accumulator: reduced
+// indices [begin, begin + count).
+struct slice_chunk
+{
+	int64_t begin;
+	int64_t count;
+};
+
 // Everything an insertion kernel reads, by value.
 struct insertion_params
 {

@@ -1,6 +1,7 @@
 // Host launchers of the kernels, called by the C ABI.
 #pragma once
 
+#include "bin_layout.hpp"
 #include "fft.hpp"
 #include "insertion.hpp"
 
@@ -15,10 +16,49 @@ enum workspace_area : int
 	area_fft = 2,
 	area_prepared = 3,
 	area_compensation = 4,
-	area_count = 5
+	area_bins = 5,
+	area_count = 6
 };
 
 void *workspace_area_bytes(int area, std::size_t bytes, hipStream_t stream);
+
+// Where an insertion takes its workspace: the per-stream areas above (grown
+// on demand), or one fixed device buffer that the whole call must fit in
+// (taken from its start, 256-byte aligned, nothing allocated).
+struct workspace_source
+{
+	char *arena = nullptr; // null: the per-stream areas
+	std::size_t arena_bytes = 0;
+	std::size_t used = 0;
+
+	void *take(int area, std::size_t bytes, hipStream_t stream)
+	{
+		if (arena == nullptr)
+		{
+			return workspace_area_bytes(area, bytes, stream);
+		}
+		const std::size_t at = (used + 255) / 256 * 256;
+		if (at > arena_bytes || bytes > arena_bytes - at)
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "workspace: the call needs " + std::to_string(at + bytes) +
+				" bytes of workspace or more, beyond the " + std::to_string(arena_bytes) +
+				" bytes of the fixed scratch buffer");
+		}
+		used = at + bytes;
+		return arena + at;
+	}
+
+	// Bytes a take() can still get (the per-stream areas: no limit).
+	std::size_t remaining() const
+	{
+		if (arena == nullptr)
+		{
+			return ~std::size_t(0);
+		}
+		const std::size_t at = (used + 255) / 256 * 256;
+		return at >= arena_bytes ? 0 : arena_bytes - at;
+	}
+};
 
 // Insertion schedules.
 enum insertion_schedule : int
@@ -41,16 +81,33 @@ struct pending_columns
 // place where the transforms run on their own).
 void run_insertion(insertion_params p, int schedule, const float2 *num_in, const int64_t *num_in_strides,
 	const float *den_in, const int64_t *den_in_strides, bool in_place, double shape, const pending_columns *columns,
-	hipStream_t stream);
+	workspace_source &workspace, hipStream_t stream);
 
-void set_brick_tuning(int extent_y, int extent_z, int share);
+// rexlib_hip_insert_ctf_slices with its workspace taken from `workspace`.
+void insert_ctf_slices_within(const rexlib_hip_array *numerator, const rexlib_hip_array *denominator,
+	const rexlib_hip_array *slices, const rexlib_hip_array *transforms, const rexlib_hip_array *ctf,
+	const rexlib_hip_array *radial_weights, const rexlib_hip_window *window, const rexlib_hip_ctf_optics *optics,
+	double pixel_size, const rexlib_hip_array *numerator_out, const rexlib_hip_array *denominator_out,
+	workspace_source &workspace, void *stream);
+
+// Brick shape (log2 of the interior extents) and records per task of the
+// binned insertion, per volume extent (0: every extent without an entry).
+void set_insertion_tuning(int extent, int shift_x, int shift_y, int shift_z, int records_per_task);
+void get_insertion_tuning(int extent, int *shift_x, int *shift_y, int *shift_z, int *records_per_task);
+void clear_insertion_tuning();
+
+// The bins of one insertion call (bins.hip).
+// `budget`: bytes for the bin arrays, the tasks and one chunk's records;
+// the chunk of slices shrinks to fit it (unusable when not one slice fits).
+bin_layout plan_bins(const insertion_params &p, std::size_t budget);
+void launch_bin_count(const insertion_params &p, slice_chunk chunk, const bin_state &s, hipStream_t stream);
+void launch_bin_brick(const insertion_params &p, slice_chunk chunk, const bin_state &s, hipStream_t stream);
 
 // Event timing of single kernels (ids as REXLIB_HIP_KERNEL_*).
 void set_kernel_timing(bool enable);
 int timing_begin(int kernel, hipStream_t stream);
 void timing_end(int token, hipStream_t stream);
 void kernel_timing(int kernel, double *total_ms, int64_t *launches);
-void get_brick_tuning(int *extent_y, int *extent_z, int *share);
 
 void launch_rfft_rows(int kind, const void *in, const line_set &in_rows, int m, const float *scales,
 	int64_t scale_stride, float2 *out, const line_set &out_rows, int64_t n, hipStream_t stream);
@@ -64,8 +121,8 @@ void launch_irfft_rows(const float2 *src, const line_set &src_rows, float *dst, 
 void launch_normalize(const float2 *num, const float *den, float2 *out, int64_t count, double threshold,
 	hipStream_t stream);
 
-void launch_gridding(const float *in, float *out, int big, int64_t volumes, int profile, double half_width,
-	double shape, hipStream_t stream);
+void launch_gridding(const float *in, float *out, int big, int crop, int64_t volumes, int profile,
+	double half_width, double shape, hipStream_t stream);
 
 int mlp_parameter_count();
 void launch_compensation(const float *den, float *w, float *scratch, int big, int passes, float floor_value,

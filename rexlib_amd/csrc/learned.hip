@@ -127,9 +127,14 @@ __global__ __launch_bounds__(256) void features_kernel(const float *den, __half 
 			}
 		}
 	}
-	// log(1 + v) by the hardware logarithm: the features are stored as
-	// float16, whose rounding (about 5e-4 relative) is far above its error.
-	auto log1 = [](float v) { return __logf(1.0f + v); };
+	// log(1 + v) by the hardware logarithm, corrected for the rounding of
+	// 1 + v: with u = fl(1 + v), log(u) v / (u - 1) keeps the relative error
+	// near one ulp for small v as well (u = 1 gives v itself).
+	auto log1 = [](float v)
+	{
+		const float u = 1.0f + v;
+		return u == 1.0f ? v : __logf(u) * (v / (u - 1.0f));
+	};
 	float f[16];
 	f[0] = sqrtf(static_cast<float>(x * x + y * y + z * z)) / (0.5f * static_cast<float>(big));
 	f[1] = log1(fmaxf(den[i], 0.0f));

@@ -55,27 +55,36 @@ __device__ double window_transform(int profile, double half_width, double shape,
 	return kb(shape * shape - t * t) / kb(shape * shape);
 }
 
-__global__ void gridding_kernel(const float *in, float *out, int big, int64_t volumes, int profile,
+// out (crop^3 per volume) = the central crop^3 of in (big^3 per volume),
+// divided by the window's transform along each axis. The crop starts at
+// big/2 - crop/2, so that the volume's centre big/2 stays at crop/2 (the
+// central n^3 of a reconstruction inserted with 2x padding).
+__global__ void gridding_kernel(const float *in, float *out, int big, int crop, int64_t volumes, int profile,
 	double half_width, double shape)
 {
 	extern __shared__ float table[];
-	for (int i = threadIdx.x; i < big; i += blockDim.x)
+	const int offset = big / 2 - crop / 2;
+	for (int i = threadIdx.x; i < crop; i += blockDim.x)
 	{
-		const double u = static_cast<double>(i - big / 2) / static_cast<double>(big);
+		const double u = static_cast<double>(i + offset - big / 2) / static_cast<double>(big);
 		table[i] = static_cast<float>(1.0 / window_transform(profile, half_width, shape, u));
 	}
 	__syncthreads();
-	const int64_t volume = static_cast<int64_t>(big) * big * big;
+	const int64_t volume = static_cast<int64_t>(crop) * crop * crop;
+	const int64_t big_volume = static_cast<int64_t>(big) * big * big;
 	const int64_t count = volume * volumes;
 	for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < count;
 		i += static_cast<int64_t>(gridDim.x) * blockDim.x)
 	{
-		int64_t v = i % volume;
-		const int x = static_cast<int>(v % big);
-		v /= big;
-		const int y = static_cast<int>(v % big);
-		const int z = static_cast<int>(v / big);
-		out[i] = in[i] * table[x] * table[y] * table[z];
+		const int64_t b = i / volume;
+		int64_t v = i - b * volume;
+		const int x = static_cast<int>(v % crop);
+		v /= crop;
+		const int y = static_cast<int>(v % crop);
+		const int z = static_cast<int>(v / crop);
+		const int64_t source = b * big_volume + ((static_cast<int64_t>(z + offset) * big + (y + offset)) * big) +
+			(x + offset);
+		out[i] = in[source] * table[x] * table[y] * table[z];
 	}
 }
 
@@ -148,11 +157,11 @@ void launch_normalize(const float2 *num, const float *den, float2 *out, int64_t 
 	REXLIB_HIP_CHECK(hipGetLastError());
 }
 
-void launch_gridding(const float *in, float *out, int big, int64_t volumes, int profile, double half_width,
-	double shape, hipStream_t stream)
+void launch_gridding(const float *in, float *out, int big, int crop, int64_t volumes, int profile,
+	double half_width, double shape, hipStream_t stream)
 {
-	const int64_t count = static_cast<int64_t>(big) * big * big * volumes;
-	gridding_kernel<<<blocks_for(count), 256, sizeof(float) * big, stream>>>(in, out, big, volumes, profile,
+	const int64_t count = static_cast<int64_t>(crop) * crop * crop * volumes;
+	gridding_kernel<<<blocks_for(count), 256, sizeof(float) * crop, stream>>>(in, out, big, crop, volumes, profile,
 		half_width, shape);
 	REXLIB_HIP_CHECK(hipGetLastError());
 }

@@ -203,11 +203,17 @@ def normalize(numerator, denominator, threshold: float = 1e-3, out=None, context
     return out
 
 
-def gridding_correct(volume, window: GriddingWindow, out=None, context: ExecutionContext | None = None):
-    """Divide a real volume (origin at N // 2) by the window's transform per axis."""
+def gridding_correct(volume, window: GriddingWindow, out=None, crop: int | None = None,
+                     context: ExecutionContext | None = None):
+    """Divide a real volume (origin at N // 2) by the window's transform per axis.
+
+    With ``crop`` = M < N (or an ``out`` of extent M), only the central M^3
+    (from N//2 - M//2) is written: the crop of a reconstruction inserted with
+    padding."""
     ctx = _context(context)
     if out is None:
-        out = torch.empty_like(volume)
+        m = volume.shape[-1] if crop is None else int(crop)
+        out = torch.empty(tuple(volume.shape[:-3]) + (m, m, m), dtype=volume.dtype, device=volume.device)
     N.check(N.library().rexlib_hip_gridding_correct(N.array(volume), window.native(), N.array(out),
                                                     ctx.handle()))
     return out

@@ -1,10 +1,13 @@
-"""Autotuner of the insertion's LDS brick schedule.
+"""Autotuner of the insertion's brick schedule.
 
 For a workload (slice extent n, volume extent N, slices per launch L) it times
-the insertion on the device for every brick shape (extents along y and z;
-the extent along x is 16; 6 along z goes with 8 along y) and share target (expected plane hits per task), and
-records the fastest in a JSON table keyed by "n,N". ``apply(n, N)`` sets the
-recorded choice in the library before an insertion.
+the insertion on the device for every brick shape the library holds (log2 of
+the brick's extents along x, y and z) and several task sizes (records per
+task), checks every configuration's sums against the first one's, and records
+the fastest per volume extent N in a JSON table. ``apply_all()`` hands the
+whole table to the library, which then picks each call's entry by its volume
+extent, so reconstructions of different sizes need no switching between
+calls.
 
     python -m rexlib_amd.tune --n 256 --N 256 --launch 4096
 """
@@ -21,9 +24,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
-EXTENTS_Y = (8, 16)
-EXTENTS_Z = (6, 8, 16)
-SHARES = (64, 96, 128, 192, 256, 512, 1024)
+# (log2 x, log2 y, log2 z) of the bricks the library is built for.
+SHAPES = ((3, 3, 3), (4, 3, 3), (3, 3, 2), (4, 3, 2))
+RECORDS_PER_TASK = (2048, 4096, 8192, 16384, 32768)
+DEFAULT = {"shift": [3, 3, 3], "records_per_task": 8192}
 
 
 def load_table(path: str = TABLE) -> dict:
@@ -33,30 +37,31 @@ def load_table(path: str = TABLE) -> dict:
         return json.load(f)
 
 
-def set_tuning(extent_y: int, extent_z: int, share: int) -> None:
+def set_tuning(extent: int, shift, records_per_task: int) -> None:
+    """The library's entry for volumes of extent ``extent`` (0: every other extent)."""
     from . import _native as N
-    N.library().rexlib_hip_set_insertion_tuning(int(extent_y), int(extent_z), int(share))
+    N.check(N.library().rexlib_hip_set_insertion_tuning(int(extent), int(shift[0]), int(shift[1]), int(shift[2]),
+                                                         int(records_per_task)))
 
 
-def get_tuning():
+def get_tuning(extent: int):
     from . import _native as N
-    w, d, s = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-    N.library().rexlib_hip_get_insertion_tuning(ctypes.byref(w), ctypes.byref(d), ctypes.byref(s))
-    return w.value, d.value, s.value
+    v = [ctypes.c_int32() for _ in range(4)]
+    N.check(N.library().rexlib_hip_get_insertion_tuning(int(extent), *[ctypes.byref(x) for x in v]))
+    return (v[0].value, v[1].value, v[2].value), v[3].value
 
 
-def apply(n: int, big: int, path: str = TABLE) -> tuple[int, int, int] | None:
-    """Set the recorded choice for (n, N), or the nearest recorded N for n."""
+def clear_tuning() -> None:
+    from . import _native as N
+    N.library().rexlib_hip_clear_insertion_tuning()
+
+
+def apply_all(path: str = TABLE) -> dict:
+    """Hand every recorded entry (keyed by N) to the library; returns them."""
     table = load_table(path)
-    key = f"{n},{big}"
-    entry = table.get(key)
-    if entry is None:
-        candidates = [(abs(int(k.split(",")[1]) - big), v) for k, v in table.items()]
-        if not candidates:
-            return None
-        entry = min(candidates, key=lambda c: c[0])[1]
-    set_tuning(entry["extent_y"], entry["extent_z"], entry["share"])
-    return entry["extent_y"], entry["extent_z"], entry["share"]
+    for key, entry in table.items():
+        set_tuning(int(key), entry["shift"], entry["records_per_task"])
+    return table
 
 
 def workload(n: int, big: int, launch: int, seed: int = 0):
@@ -73,7 +78,8 @@ def workload(n: int, big: int, launch: int, seed: int = 0):
     return slices, transforms, ctf
 
 
-def time_insertion(n, big, slices, transforms, ctf, repeats=3) -> float:
+def time_insertion(n, big, slices, transforms, ctf, repeats=3):
+    """Best time of a CTF insertion into zeroed accumulators, and its sums."""
     import torch
 
     from . import em
@@ -90,6 +96,7 @@ def time_insertion(n, big, slices, transforms, ctf, repeats=3) -> float:
 
     run()
     torch.cuda.synchronize()
+    result = (num.clone(), den.clone())
     best = float("inf")
     for _ in range(repeats):
         a = torch.cuda.Event(enable_timing=True)
@@ -99,30 +106,44 @@ def time_insertion(n, big, slices, transforms, ctf, repeats=3) -> float:
         b.record()
         torch.cuda.synchronize()
         best = min(best, a.elapsed_time(b))
-    return best
+    return best, result
+
+
+def rel_l2(a, b) -> float:
+    import torch
+    return float(torch.linalg.vector_norm(a - b) / torch.clamp(torch.linalg.vector_norm(b), min=1e-30))
 
 
 def tune(n: int, big: int, launch: int, path: str = TABLE, verbose: bool = True) -> dict:
     slices, transforms, ctf = workload(n, big, launch)
     results = []
-    for ey in EXTENTS_Y:
-        for ez in EXTENTS_Z:
-            if ez == 6 and ey != 8:
-                continue  # 6 along z is built for 8 along y only
-            for share in SHARES:
-                set_tuning(ey, ez, share)
-                ms = time_insertion(n, big, slices, transforms, ctf)
-                results.append({"extent_y": ey, "extent_z": ez, "share": share, "ms": ms})
+    reference = None
+    previous = get_tuning(big)
+    try:
+        for shift in SHAPES:
+            for per_task in RECORDS_PER_TASK:
+                set_tuning(big, shift, per_task)
+                ms, sums = time_insertion(n, big, slices, transforms, ctf)
+                if reference is None:
+                    reference = sums
+                error = max(rel_l2(sums[0], reference[0]), rel_l2(sums[1], reference[1]))
+                if not error <= 1e-5:
+                    raise RuntimeError(f"brick shape {shift}, {per_task} records per task: sums differ from "
+                                       f"the first configuration's by rel-L2 {error:.3e}")
+                results.append({"shift": list(shift), "records_per_task": per_task, "ms": ms, "rel_l2": error})
                 if verbose:
-                    print(f"n={n} N={big} L={launch} extent_y={ey} extent_z={ez} share={share}: {ms:.3f} ms",
-                          flush=True)
+                    print(f"n={n} N={big} L={launch} shift={shift} records/task={per_task}: {ms:.3f} ms "
+                          f"(rel-L2 to the first {error:.1e})", flush=True)
+    finally:
+        set_tuning(big, *previous)
     best = min(results, key=lambda r: r["ms"])
     table = load_table(path)
-    table[f"{n},{big}"] = {**best, "launch": launch, "images_per_s": launch / (best["ms"] * 1e-3),
-                           "tuned": time.strftime("%Y-%m-%d")}
+    table[str(big)] = {"shift": best["shift"], "records_per_task": best["records_per_task"], "n": n,
+                       "launch": launch, "ms": best["ms"], "images_per_s": launch / (best["ms"] * 1e-3),
+                       "tuned": time.strftime("%Y-%m-%d")}
     with open(path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
-    set_tuning(best["extent_y"], best["extent_z"], best["share"])
+    set_tuning(big, best["shift"], best["records_per_task"])
     return {"best": best, "all": results}
 
 

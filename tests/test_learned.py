@@ -90,3 +90,15 @@ def test_features_at_the_origin():
     assert f[0, 0, 0, 1] == pytest.approx(1.0)
     # The neighbour (1, 0, 0) sees the origin in the octants with ox = -1.
     assert f[0, 0, 1, 2] == pytest.approx(np.log1p((np.e - 1) / 7))
+
+
+def test_separable_k3_equals_the_27_point_sum():
+    field = np.random.default_rng(3).normal(size=(6, 7, 5))
+    np.testing.assert_allclose(oracle.convolve_k3(field), oracle.convolve_k3_direct(field), rtol=1e-13, atol=1e-13)
+    # A delta spreads into the kernel itself: 2/3 at the centre, (1/6)^3 in the corners.
+    delta = np.zeros((5, 5, 5))
+    delta[2, 2, 2] = 1.0
+    c = oracle.convolve_k3(delta)
+    assert c[2, 2, 2] == pytest.approx(8 / 27)
+    assert c[1, 1, 1] == pytest.approx(1 / 216)
+    assert c.sum() == pytest.approx(1.0)

@@ -139,3 +139,45 @@ def test_python_api_rejects_host_tensors():
     with pytest.raises(ValueError):
         em.CtfOptics(300, 2.7, 1.5)
     assert em.CtfOptics(300, 2.7, 0.1).wavelength == pytest.approx(0.019687615309, rel=1e-9)
+
+
+def strided(shape, strides, dtype, address=0x1000):
+    a = fake(shape, dtype, address)
+    for i, s in enumerate(strides):
+        a.strides[i] = s
+    return a
+
+
+def test_gridding_correction_checks_its_operands():
+    lib = N.library()
+    window = N.Window(1, 1.0, 0.0)
+
+    def call(volume, out):
+        return lib.rexlib_hip_gridding_correct(ctypes.byref(volume), ctypes.byref(window), ctypes.byref(out), None)
+
+    # A (4, 4, 64) volume is not a cube: rejected before any memory is read.
+    assert call(fake((4, 4, 64), N.FLOAT32), fake((4, 4, 64), N.FLOAT32)) == N.INVALID_ARGUMENT
+    # A crop larger than the volume, and a crop aliasing its input.
+    assert call(fake((8, 8, 8), N.FLOAT32), fake((16, 16, 16), N.FLOAT32, 0x2000)) == N.INVALID_ARGUMENT
+    assert call(fake((16, 16, 16), N.FLOAT32), fake((8, 8, 8), N.FLOAT32)) == N.INVALID_ARGUMENT
+    # Batch shapes differ.
+    assert call(fake((2, 8, 8, 8), N.FLOAT32), fake((3, 8, 8, 8), N.FLOAT32, 0x2000)) == N.INVALID_ARGUMENT
+    # A transposed view has no gfx950 program.
+    assert call(strided((8, 8, 8), (1, 8, 64), N.FLOAT32), fake((8, 8, 8), N.FLOAT32, 0x2000)) == N.UNSUPPORTED
+    assert "contiguous" in message()
+
+
+def test_normalize_and_shell_sums_check_their_operands():
+    lib = N.library()
+    num = fake((8, 8, 5), N.COMPLEX64)
+    den = fake((8, 8, 5), N.FLOAT32, 0x2000)
+    out = strided((8, 8, 5), (1, 5, 40), N.COMPLEX64, 0x3000)
+    assert lib.rexlib_hip_normalize(ctypes.byref(num), ctypes.byref(den), 1e-3, ctypes.byref(out), None) == \
+        N.UNSUPPORTED
+    bad = fake((8, 8, 8), N.COMPLEX64)
+    assert lib.rexlib_hip_normalize(ctypes.byref(bad), ctypes.byref(den), 1e-3, ctypes.byref(bad), None) == \
+        N.INVALID_ARGUMENT
+    b = strided((8, 8, 5), (40, 1, 8), N.COMPLEX64, 0x2000)
+    assert lib.rexlib_hip_shell_sums(ctypes.byref(num), ctypes.byref(b), None, None) == N.UNSUPPORTED
+    batched = fake((2, 8, 8, 5), N.COMPLEX64)
+    assert lib.rexlib_hip_shell_sums(ctypes.byref(batched), ctypes.byref(batched), None, None) == N.UNSUPPORTED
