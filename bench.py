@@ -57,7 +57,8 @@ def traffic_per_launch(args, tuned):
     if record.get("n") != args.n or record.get("N") != args.big or record.get("launch") != args.launch:
         return None
     brick = record.get("brick", {})
-    if tuned is None or (tuple(brick.get("shift", ())), brick.get("records_per_task")) != (tuple(tuned[0]), tuned[1]):
+    if tuned is None or ((brick.get("extent_y"), brick.get("extent_z")), brick.get("share")) != (tuple(tuned[0]),
+                                                                                              tuned[1]):
         return None  # measured for another brick shape
     return record["bytes_per_launch"]
 
@@ -343,7 +344,7 @@ def main():
     # entry of this box size.
     tune.apply_all()
     tuned = tune.get_tuning(big)
-    log(f"brick tuning (log2 brick extents, records per task): {tuned}")
+    log(f"brick tuning ((extent y, extent z), share): {tuned}")
 
     def step():
         num.zero_()
@@ -379,7 +380,6 @@ def main():
     elapsed = time.perf_counter() - t0
     brick_ms, brick_launches = _native.kernel_timing(_native.KERNEL_BRICK)
     prepare_ms, prepare_launches = _native.kernel_timing(_native.KERNEL_PREPARE)
-    count_ms, count_launches = _native.kernel_timing(_native.KERNEL_BIN_COUNT)
     _native.set_kernel_timing(False)
     ms_per_step = 1e3 * elapsed / args.steps
     value = args.images * args.steps / elapsed
@@ -397,7 +397,7 @@ def main():
     achieved = b_img * images_timed / (brick_ms * 1e-3) / 1e9 if brick_launches else None
     roofline = {
         "bound": "hbm",
-        "kernel": "bin_brick_kernel (binned LDS brick insertion, CTF + trilinear)",
+        "kernel": "brick_kernel (tile-owned LDS brick gather, CTF + trilinear)",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -408,7 +408,6 @@ def main():
         "images_per_launch": images_per_launch,
         "bytes_per_image": b_img,
         "column_prepare_kernel_ms": prepare_ms / prepare_launches if prepare_launches else None,
-        "bin_count_and_plan_ms": count_ms / count_launches if count_launches else None,
         "pipeline_frac": value * b_img / (HBM_PEAK_GBS * 1e9),
     }
 
@@ -449,7 +448,7 @@ def main():
             "workload": (f"plain FP32: {args.images} images {n}^2 -> {big}^3, launches of {L}, fused rfft2 + "
                          "CTF trilinear insertion, then quotient, irfft3, gridding correction"),
             "images": args.images, "n": n, "N": big, "launch": L, "window": "linear", "variant": "ctf",
-            "brick": {"shift": list(tuned[0]), "records_per_task": tuned[1]} if tuned else None,
+            "brick": {"extent_y": tuned[0][0], "extent_z": tuned[0][1], "share": tuned[1]} if tuned else None,
         },
         "roofline": roofline,
         "cpu_baseline": baseline,

@@ -42,7 +42,8 @@ typedef enum rexlib_hip_dtype {
 	REXLIB_HIP_COMPLEX64 = 2,
 	REXLIB_HIP_COMPLEX128 = 3,
 	REXLIB_HIP_FLOAT16 = 4,
-	REXLIB_HIP_BFLOAT16 = 5
+	REXLIB_HIP_BFLOAT16 = 5,
+	REXLIB_HIP_BOOL = 6 /* one byte, 0 or 1 */
 } rexlib_hip_dtype;
 
 /* A strided view of device memory: the address of element (0, ..., 0),
@@ -94,19 +95,17 @@ void rexlib_hip_version(int32_t *major, int32_t *minor, int32_t *patch);
 int32_t rexlib_hip_device_count(void);
 
 /* Tuning of the brick schedule of the insertions, per volume extent N
- * (extent 0: every extent without an entry of its own): the brick of
- * first-tap cells a wavefront owns in LDS, as log2 of its extents along x, y
- * and z -- (3, 3, 3), (4, 3, 3), (3, 3, 2) or (4, 3, 2) -- and the records
- * (sample copies) per task, in [256, 2^24], that splits busy bricks. Each
+ * (extent 0: every extent without an entry of its own): the extents along y
+ * and z of the brick a wavefront owns in LDS (16 along x) -- (8, 6), (8, 8),
+ * (8, 16), (16, 8) or (16, 16) -- and the expected plane hits per task, in
+ * [16, 2^20], that splits busy bricks into shares of their slices. Each
  * call reads its extent's entry once; without entries the default is
- * (3, 3, 3) and 8192. Returns REXLIB_HIP_INVALID_ARGUMENT for other values.
+ * (8, 6) and 256. Returns REXLIB_HIP_INVALID_ARGUMENT for other values.
  * No reference counterpart (the reference's CPU tuning is
  * slice_insertion_tuning, src/backends/cpu/builders/em/reconstruction/slice_insertion_tuning.cpp:10-13);
  * set by the autotuner (rexlib_amd/tune.py). */
-int rexlib_hip_set_insertion_tuning(int32_t extent, int32_t shift_x, int32_t shift_y, int32_t shift_z,
-	int32_t records_per_task);
-int rexlib_hip_get_insertion_tuning(int32_t extent, int32_t *shift_x, int32_t *shift_y, int32_t *shift_z,
-	int32_t *records_per_task);
+int rexlib_hip_set_insertion_tuning(int32_t extent, int32_t extent_y, int32_t extent_z, int32_t share);
+int rexlib_hip_get_insertion_tuning(int32_t extent, int32_t *extent_y, int32_t *extent_z, int32_t *share);
 /* Drops every entry (the defaults apply again). */
 void rexlib_hip_clear_insertion_tuning(void);
 
@@ -121,11 +120,10 @@ void rexlib_hip_clear_insertion_tuning(void);
 typedef enum rexlib_hip_kernel
 {
     REXLIB_HIP_KERNEL_BRICK = 0,   /* brick insertion kernel */
-    REXLIB_HIP_KERNEL_PREPARE = 1, /* per-sample preparation writing the brick schedule's bins;
-                                      for images through rexlib_hip_insert_ctf_images, the
-                                      column transforms fused with it */
-    REXLIB_HIP_KERNEL_SCATTER = 2, /* atomic scatter insertion kernel */
-    REXLIB_HIP_KERNEL_BIN_COUNT = 3 /* counting pass and bin plan of the brick schedule */
+    REXLIB_HIP_KERNEL_PREPARE = 1, /* per-sample preparation for the brick kernel; for images
+                                      through rexlib_hip_insert_ctf_images, the column
+                                      transforms fused with it */
+    REXLIB_HIP_KERNEL_SCATTER = 2  /* atomic scatter insertion kernel */
 } rexlib_hip_kernel;
 
 int rexlib_hip_set_kernel_timing(int32_t enable);
@@ -268,6 +266,64 @@ int rexlib_hip_normalize(const rexlib_hip_array *numerator, const rexlib_hip_arr
  * it may then alias volume. Contiguous operands. */
 int rexlib_hip_gridding_correct(const rexlib_hip_array *volume, const rexlib_hip_window *window,
 	const rexlib_hip_array *out, void *stream);
+
+/* ------------------------------------------------------------------------- */
+/* General transforms: any axes (in the given order; the last one listed is  */
+/* the halved axis of rfft / irfft), any strides, lengths whose prime factors */
+/* are 2, 3 and 5 up to 4096 directly and other lengths up to 2048 by        */
+/* Bluestein's chirp-z algorithm. normalization: 0 backward (inverse scaled  */
+/* by 1/count), 1 ortho (both by 1/sqrt(count)), 2 forward (forward scaled). */
+/* float32 / complex64; float64 operands are UNSUPPORTED (the reference's    */
+/* CPU backend keeps them). out may alias in.                                 */
+/* Replace the programs behind ops::fft_operation, ifft_operation,            */
+/* rfft_operation and irfft_operation (include/rexlib/ops/fourier/*.hpp; CPU */
+/* src/backends/cpu/kernels/pocketfft_fourier_transform.cpp:74-163).          */
+/* ------------------------------------------------------------------------- */
+
+/* Complex-to-complex DFT over `axes`: forward (exp(-i...)) or inverse. */
+int rexlib_hip_fft(const rexlib_hip_array *in, const rexlib_hip_array *out, const int32_t *axes,
+	int32_t axis_count, int32_t inverse, int32_t normalization, void *stream);
+
+/* Real-to-complex DFT over `axes`; out holds n/2 + 1 along the last axis listed. */
+int rexlib_hip_rfft(const rexlib_hip_array *in, const rexlib_hip_array *out, const int32_t *axes,
+	int32_t axis_count, int32_t normalization, void *stream);
+
+/* Complex-to-real inverse DFT over `axes`; in holds n/2 + 1 along the last axis
+ * listed, out's extent n there; the imaginary parts of self-conjugate
+ * coefficients are ignored. */
+int rexlib_hip_irfft(const rexlib_hip_array *in, const rexlib_hip_array *out, const int32_t *axes,
+	int32_t axis_count, int32_t normalization, void *stream);
+
+/* ------------------------------------------------------------------------- */
+/* Elementwise programs the quotient is written with. Operands broadcast     */
+/* against the result as numpy's (an axis of extent 1, or a missing leading  */
+/* axis, repeats); any strides.                                               */
+/* ------------------------------------------------------------------------- */
+
+/* Every element of out = real (+ i imaginary for complex types); float32,
+ * float64, complex64, complex128, float16, bfloat16 and bool (real != 0).
+ * Replaces the program behind ops::fill_operation (src/functional/creation.cpp:301-349). */
+int rexlib_hip_fill(const rexlib_hip_array *out, double real, double imaginary, void *stream);
+
+/* result = dividend / divisor, one element type: float32 or complex64
+ * (complex quotients scale by the divisor's larger part first).
+ * Replaces the program behind ops::divide_operation
+ * (include/rexlib/ops/arithmetic/divide_operation.hpp; CPU
+ * src/backends/cpu/builders/arithmetic/divide_program_builder.cpp). */
+int rexlib_hip_divide(const rexlib_hip_array *dividend, const rexlib_hip_array *divisor,
+	const rexlib_hip_array *result, void *stream);
+
+/* result (bool) = a > b, float32 operands.
+ * Replaces the program behind ops::greater_operation
+ * (include/rexlib/ops/compare/greater_operation.hpp). */
+int rexlib_hip_greater(const rexlib_hip_array *a, const rexlib_hip_array *b, const rexlib_hip_array *result,
+	void *stream);
+
+/* result = condition ? x : y; condition bool, x, y and result one type,
+ * float32 or complex64. Replaces the program behind ops::where_operation
+ * (include/rexlib/ops/selection/where_operation.hpp). */
+int rexlib_hip_where(const rexlib_hip_array *condition, const rexlib_hip_array *x, const rexlib_hip_array *y,
+	const rexlib_hip_array *result, void *stream);
 
 /* Per-shell sums, in float64, of two half spectra a, b (N, N, N/2+1) complex64:
  * sums has 4 * (N/2+1) doubles = [sum Re(a conj b), sum |a|^2, sum |b|^2, count]

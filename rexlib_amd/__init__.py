@@ -5,8 +5,10 @@ rexlib_amd/build.py. This package is the host-side mirror of rexlib's Python
 API for the path: ``rexlib_amd.em.insert_*_slices``, ``rexlib_amd.rfft2``,
 ``rexlib_amd.irfft3`` and the reconstruction steps.
 """
-from . import em, fourier
+from . import em, fourier, functional
 from ._native import UnsupportedOperationError, library
 from .fourier import irfft2, irfft3, rfft2
+from .functional import divide, fill_, full, greater, where
 
-__all__ = ["em", "fourier", "rfft2", "irfft2", "irfft3", "library", "UnsupportedOperationError"]
+__all__ = ["em", "fourier", "functional", "rfft2", "irfft2", "irfft3", "divide", "fill_", "full", "greater", "where",
+           "library", "UnsupportedOperationError"]

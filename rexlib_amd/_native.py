@@ -15,7 +15,7 @@ LIBRARY = os.path.join(HERE, "librexlib_hip.so")
 MAX_RANK = 8
 
 OK, INVALID_ARGUMENT, UNSUPPORTED, RUNTIME_ERROR = 0, 1, 2, 3
-FLOAT32, FLOAT64, COMPLEX64, COMPLEX128, FLOAT16, BFLOAT16 = 0, 1, 2, 3, 4, 5
+FLOAT32, FLOAT64, COMPLEX64, COMPLEX128, FLOAT16, BFLOAT16, BOOL = 0, 1, 2, 3, 4, 5, 6
 
 _DTYPES = {
     torch.float32: FLOAT32,
@@ -24,6 +24,7 @@ _DTYPES = {
     torch.complex128: COMPLEX128,
     torch.float16: FLOAT16,
     torch.bfloat16: BFLOAT16,
+    torch.bool: BOOL,
 }
 
 
@@ -68,6 +69,13 @@ EXPORTS = [
     "rexlib_hip_normalize",
     "rexlib_hip_gridding_correct",
     "rexlib_hip_shell_sums",
+    "rexlib_hip_fft",
+    "rexlib_hip_rfft",
+    "rexlib_hip_irfft",
+    "rexlib_hip_fill",
+    "rexlib_hip_divide",
+    "rexlib_hip_greater",
+    "rexlib_hip_where",
     "rexlib_hip_density_compensation",
     "rexlib_hip_compensation_features",
     "rexlib_hip_mlp_parameter_count",
@@ -100,9 +108,9 @@ def library() -> ctypes.CDLL:
     lib.rexlib_hip_version.restype = None
     lib.rexlib_hip_device_count.restype = ctypes.c_int32
     lib.rexlib_hip_device_count.argtypes = []
-    lib.rexlib_hip_set_insertion_tuning.argtypes = [ctypes.c_int32] * 5
+    lib.rexlib_hip_set_insertion_tuning.argtypes = [ctypes.c_int32] * 4
     lib.rexlib_hip_set_insertion_tuning.restype = ctypes.c_int
-    lib.rexlib_hip_get_insertion_tuning.argtypes = [ctypes.c_int32] + [P(ctypes.c_int32)] * 4
+    lib.rexlib_hip_get_insertion_tuning.argtypes = [ctypes.c_int32] + [P(ctypes.c_int32)] * 3
     lib.rexlib_hip_get_insertion_tuning.restype = ctypes.c_int
     lib.rexlib_hip_clear_insertion_tuning.argtypes = []
     lib.rexlib_hip_clear_insertion_tuning.restype = None
@@ -116,6 +124,14 @@ def library() -> ctypes.CDLL:
     lib.rexlib_hip_normalize.argtypes = [A, A, ctypes.c_double, A, vp]
     lib.rexlib_hip_gridding_correct.argtypes = [A, W, A, vp]
     lib.rexlib_hip_shell_sums.argtypes = [A, A, vp, vp]
+    lib.rexlib_hip_fill.argtypes = [A, ctypes.c_double, ctypes.c_double, vp]
+    I32 = ctypes.c_int32
+    lib.rexlib_hip_fft.argtypes = [A, A, P(I32), I32, I32, I32, vp]
+    lib.rexlib_hip_rfft.argtypes = [A, A, P(I32), I32, I32, vp]
+    lib.rexlib_hip_irfft.argtypes = [A, A, P(I32), I32, I32, vp]
+    lib.rexlib_hip_divide.argtypes = [A, A, A, vp]
+    lib.rexlib_hip_greater.argtypes = [A, A, A, vp]
+    lib.rexlib_hip_where.argtypes = [A, A, A, A, vp]
     d = ctypes.c_double
     lib.rexlib_hip_density_compensation.argtypes = [A, ctypes.c_int32, d, A, vp]
     lib.rexlib_hip_compensation_features.argtypes = [A, A, vp]
@@ -165,7 +181,7 @@ def check(status: int) -> None:
     raise RuntimeError(message)
 
 
-KERNEL_BRICK, KERNEL_PREPARE, KERNEL_SCATTER, KERNEL_BIN_COUNT = 0, 1, 2, 3
+KERNEL_BRICK, KERNEL_PREPARE, KERNEL_SCATTER = 0, 1, 2
 
 
 def set_kernel_timing(enable: bool) -> None:

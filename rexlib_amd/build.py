@@ -6,9 +6,12 @@
 - rexlib_amd/librexlib_host.so: the C++ core of the mirrored rexlib API
   (rexlib_amd/host/core.cpp): services, plugin loader, arrays, the eager
   dispatcher and the em / Fourier functions.
-- rexlib_amd/librexlib-hip.so: the plugin (rexlib_amd/host/hip_plugin.cpp),
-  exporting rexlib_get_plugin_compatibility and rexlib_get_plugin; it runs
-  librexlib_hip.so's kernels through the C ABI.
+- rexlib_amd/rexlib-plugins/librexlib-hip.so: the plugin
+  (rexlib_amd/host/hip_plugin.cpp), exporting rexlib_get_plugin_compatibility
+  and rexlib_get_plugin; it runs librexlib_hip.so's kernels through the C ABI.
+  It sits in a directory named rexlib-plugins inside the package, where
+  rexlib's Python discovery looks for the plugins a distribution declares in
+  the rexlib.plugins entry-point group (pyproject.toml).
 - tests/plugin_driver: the C++ caller the plugin tests run.
 """
 from __future__ import annotations
@@ -51,7 +54,8 @@ def sources():
 ROOT = os.path.dirname(HERE)
 HOST = os.path.join(HERE, "host")
 HOST_LIBRARY = os.path.join(HERE, "librexlib_host.so")
-PLUGIN = os.path.join(HERE, "librexlib-hip.so")
+PLUGIN_DIR = os.path.join(HERE, "rexlib-plugins")
+PLUGIN = os.path.join(PLUGIN_DIR, "librexlib-hip.so")
 DRIVER = os.path.join(ROOT, "tests", "plugin_driver")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 CXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wextra", "-Wno-unused-parameter"]
@@ -72,10 +76,11 @@ def build_host(verbose: bool = False) -> list:
     _run([cxx, *CXX_FLAGS, "-fPIC", "-shared", "-fvisibility=hidden", "-I", HOST,
           os.path.join(HOST, "core.cpp"), "-ldl", "-o", HOST_LIBRARY + ".tmp"], "librexlib_host.so", verbose)
     os.replace(HOST_LIBRARY + ".tmp", HOST_LIBRARY)
+    os.makedirs(PLUGIN_DIR, exist_ok=True)
     _run([cxx, *CXX_FLAGS, "-fPIC", "-shared", "-fvisibility=hidden", "-D__HIP_PLATFORM_AMD__",
           "-I", HOST, "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROCM, "include"),
           os.path.join(HOST, "hip_plugin.cpp"), "-L", HERE, "-lrexlib_host", "-l:librexlib_hip.so",
-          "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM}/lib",
+          "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{ROCM}/lib",
           "-Wl,--no-undefined", "-o", PLUGIN + ".tmp"], "librexlib-hip.so", verbose)
     os.replace(PLUGIN + ".tmp", PLUGIN)
     _run([cxx, *CXX_FLAGS, "-I", HOST, os.path.join(ROOT, "tests", "plugin_driver.cpp"), "-L", HERE,

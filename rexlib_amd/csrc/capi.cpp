@@ -490,27 +490,24 @@ void rexlib_hip_version(int32_t *major, int32_t *minor, int32_t *patch)
 	}
 }
 
-int rexlib_hip_set_insertion_tuning(int32_t extent, int32_t shift_x, int32_t shift_y, int32_t shift_z,
-	int32_t records_per_task)
+int rexlib_hip_set_insertion_tuning(int32_t extent, int32_t extent_y, int32_t extent_z, int32_t share)
 {
-	return guarded([&] { rexlib_hip::set_insertion_tuning(extent, shift_x, shift_y, shift_z, records_per_task); });
+	return guarded([&] { rexlib_hip::set_insertion_tuning(extent, extent_y, extent_z, share); });
 }
 
-int rexlib_hip_get_insertion_tuning(int32_t extent, int32_t *shift_x, int32_t *shift_y, int32_t *shift_z,
-	int32_t *records_per_task)
+int rexlib_hip_get_insertion_tuning(int32_t extent, int32_t *extent_y, int32_t *extent_z, int32_t *share)
 {
 	return guarded([&]
 		{
-			if (!shift_x || !shift_y || !shift_z || !records_per_task)
+			if (!extent_y || !extent_z || !share)
 			{
 				throw rexlib_hip::error(REXLIB_HIP_INVALID_ARGUMENT, "get_insertion_tuning: null output");
 			}
-			int sx = 0, sy = 0, sz = 0, r = 0;
-			rexlib_hip::get_insertion_tuning(extent, &sx, &sy, &sz, &r);
-			*shift_x = sx;
-			*shift_y = sy;
-			*shift_z = sz;
-			*records_per_task = r;
+			int ey = 0, ez = 0, sh = 0;
+			rexlib_hip::get_insertion_tuning(extent, &ey, &ez, &sh);
+			*extent_y = ey;
+			*extent_z = ez;
+			*share = sh;
 		});
 }
 
@@ -532,7 +529,7 @@ int rexlib_hip_kernel_timing(int32_t kernel, double *total_ms, int64_t *launches
 			{
 				throw rexlib_hip::error(REXLIB_HIP_INVALID_ARGUMENT, "kernel_timing: null output");
 			}
-			if (kernel < REXLIB_HIP_KERNEL_BRICK || kernel > REXLIB_HIP_KERNEL_BIN_COUNT)
+			if (kernel < REXLIB_HIP_KERNEL_BRICK || kernel > REXLIB_HIP_KERNEL_SCATTER)
 			{
 				throw rexlib_hip::error(REXLIB_HIP_INVALID_ARGUMENT, "kernel_timing: unknown kernel id");
 			}

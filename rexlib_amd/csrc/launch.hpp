@@ -1,7 +1,6 @@
 // Host launchers of the kernels, called by the C ABI.
 #pragma once
 
-#include "bin_layout.hpp"
 #include "fft.hpp"
 #include "insertion.hpp"
 
@@ -16,8 +15,7 @@ enum workspace_area : int
 	area_fft = 2,
 	area_prepared = 3,
 	area_compensation = 4,
-	area_bins = 5,
-	area_count = 6
+	area_count = 5
 };
 
 void *workspace_area_bytes(int area, std::size_t bytes, hipStream_t stream);
@@ -90,18 +88,11 @@ void insert_ctf_slices_within(const rexlib_hip_array *numerator, const rexlib_hi
 	double pixel_size, const rexlib_hip_array *numerator_out, const rexlib_hip_array *denominator_out,
 	workspace_source &workspace, void *stream);
 
-// Brick shape (log2 of the interior extents) and records per task of the
-// binned insertion, per volume extent (0: every extent without an entry).
-void set_insertion_tuning(int extent, int shift_x, int shift_y, int shift_z, int records_per_task);
-void get_insertion_tuning(int extent, int *shift_x, int *shift_y, int *shift_z, int *records_per_task);
+// Brick extents along y and z and the plane hits per task of the brick
+// schedule, per volume extent (0: every extent without an entry).
+void set_insertion_tuning(int extent, int extent_y, int extent_z, int share);
+void get_insertion_tuning(int extent, int *extent_y, int *extent_z, int *share);
 void clear_insertion_tuning();
-
-// The bins of one insertion call (bins.hip).
-// `budget`: bytes for the bin arrays, the tasks and one chunk's records;
-// the chunk of slices shrinks to fit it (unusable when not one slice fits).
-bin_layout plan_bins(const insertion_params &p, std::size_t budget);
-void launch_bin_count(const insertion_params &p, slice_chunk chunk, const bin_state &s, hipStream_t stream);
-void launch_bin_brick(const insertion_params &p, slice_chunk chunk, const bin_state &s, hipStream_t stream);
 
 // Event timing of single kernels (ids as REXLIB_HIP_KERNEL_*).
 void set_kernel_timing(bool enable);

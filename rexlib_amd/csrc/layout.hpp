@@ -59,6 +59,7 @@ inline const char *dtype_name(int dtype)
 	case REXLIB_HIP_COMPLEX128: return "complex128";
 	case REXLIB_HIP_FLOAT16: return "float16";
 	case REXLIB_HIP_BFLOAT16: return "bfloat16";
+	case REXLIB_HIP_BOOL: return "bool";
 	default: return "unknown";
 	}
 }

@@ -15,19 +15,16 @@ constexpr int max_taps = 8;
 // ---------------------------------------------------------------------------
 
 // Round-half-up |k| of an integer frequency, exactly: the largest s with
-// 4 s^2 - 4 s + 1 <= 4 |k|^2 (row_insertion_kernel.inl:180-220).
+// 4 s^2 - 4 s + 1 <= 4 |k|^2 (row_insertion_kernel.inl:180-220). The float
+// estimate floor(sqrt(|k|^2) + 1/2) is within one of it for every |k|^2
+// below 2^31, so one step each way, by selects, makes it exact.
 __device__ inline int shell_of(int kx, int ky)
 {
-	const int q4 = 4 * (kx * kx + ky * ky);
-	int s = static_cast<int>(sqrtf(static_cast<float>(kx * kx + ky * ky)) + 0.5f);
-	while (4 * s * s + 4 * s + 1 <= q4)
-	{
-		++s;
-	}
-	while (s > 0 && 4 * s * s - 4 * s + 1 > q4)
-	{
-		--s;
-	}
+	const int q = kx * kx + ky * ky;
+	const int q4 = 4 * q;
+	int s = static_cast<int>(sqrtf(static_cast<float>(q)) + 0.5f);
+	s += (4 * s * s + 4 * s + 1 <= q4) ? 1 : 0;
+	s -= (s > 0 && 4 * s * s - 4 * s + 1 > q4) ? 1 : 0;
 	return s;
 }
 
@@ -192,6 +189,7 @@ __device__ inline bool prepare_sample(const insertion_params &p, const slice_ent
 // Volume position of sample k_x of row k_y with the sign applied.
 __device__ inline void sample_position(const slice_entry &e, int kx, int ky, float sign, float *c)
 {
+#pragma clang fp contract(off)
 	const float fx = static_cast<float>(kx);
 	for (int a = 0; a < 3; ++a)
 	{

@@ -43,3 +43,65 @@ def irfft3(spectrum: torch.Tensor, out: torch.Tensor | None = None, stream=None)
         out = torch.empty(spectrum.shape[:-3] + (big, big, big), dtype=torch.float32, device=spectrum.device)
     N.check(N.library().rexlib_hip_irfft3(N.array(spectrum), N.array(out), _stream(stream)))
     return out
+
+
+NORMALIZATIONS = {"backward": 0, "ortho": 1, "forward": 2}
+
+
+def _axes(axes, rank):
+    import ctypes
+    axes = tuple(range(rank)) if axes is None else tuple(int(a) for a in axes)
+    return (ctypes.c_int32 * len(axes))(*axes), len(axes), tuple(a % rank for a in axes)
+
+
+def fftn(x: torch.Tensor, axes=None, norm: str = "backward", out: torch.Tensor | None = None,
+         stream=None) -> torch.Tensor:
+    """Complex DFT over `axes` (all by default), as numpy.fft.fftn."""
+    arr, count, _ = _axes(axes, x.dim())
+    if out is None:
+        out = torch.empty(x.shape, dtype=torch.complex64, device=x.device)
+    N.check(N.library().rexlib_hip_fft(N.array(x), N.array(out), arr, count, 0, NORMALIZATIONS[norm],
+                                       _stream(stream)))
+    return out
+
+
+def ifftn(x: torch.Tensor, axes=None, norm: str = "backward", out: torch.Tensor | None = None,
+          stream=None) -> torch.Tensor:
+    """Inverse complex DFT over `axes`, as numpy.fft.ifftn."""
+    arr, count, _ = _axes(axes, x.dim())
+    if out is None:
+        out = torch.empty(x.shape, dtype=torch.complex64, device=x.device)
+    N.check(N.library().rexlib_hip_fft(N.array(x), N.array(out), arr, count, 1, NORMALIZATIONS[norm],
+                                       _stream(stream)))
+    return out
+
+
+def rfftn(x: torch.Tensor, axes=None, norm: str = "backward", out: torch.Tensor | None = None,
+          stream=None) -> torch.Tensor:
+    """Real-to-complex DFT over `axes`, the last one halved, as numpy.fft.rfftn."""
+    arr, count, normal = _axes(axes, x.dim())
+    if out is None:
+        shape = list(x.shape)
+        shape[normal[-1]] = shape[normal[-1]] // 2 + 1
+        out = torch.empty(shape, dtype=torch.complex64, device=x.device)
+    N.check(N.library().rexlib_hip_rfft(N.array(x), N.array(out), arr, count, NORMALIZATIONS[norm],
+                                        _stream(stream)))
+    return out
+
+
+def irfftn(x: torch.Tensor, s=None, axes=None, norm: str = "backward", out: torch.Tensor | None = None,
+           stream=None) -> torch.Tensor:
+    """Complex-to-real inverse DFT over `axes`, as numpy.fft.irfftn; `s` the
+    output extents along `axes` (default: 2 (m - 1) along the last one)."""
+    arr, count, normal = _axes(axes, x.dim())
+    if out is None:
+        shape = list(x.shape)
+        if s is None:
+            shape[normal[-1]] = 2 * (shape[normal[-1]] - 1)
+        else:
+            for a, e in zip(normal, s):
+                shape[a] = int(e)
+        out = torch.empty(shape, dtype=torch.float32, device=x.device)
+    N.check(N.library().rexlib_hip_irfft(N.array(x), N.array(out), arr, count, NORMALIZATIONS[norm],
+                                         _stream(stream)))
+    return out

@@ -801,6 +801,97 @@ array to_host(const_array_ref x, const execution_context &context)
 }
 
 // ---------------------------------------------------------------------------
+// Elementwise functions
+// ---------------------------------------------------------------------------
+
+array full(std::vector<std::size_t> extents, double value, numerical_type type, const execution_context &context)
+{
+	check_context(context);
+	array out;
+	const array_descriptor planned(strided_layout::make_contiguous(std::move(extents)), type);
+	context.get_dispatcher()->dispatch(ops::fill_operation(value), {&out, 1}, {&planned, 1}, {},
+		context.get_device_context());
+	return out;
+}
+
+namespace
+{
+
+// Extents of operands broadcast together, numpy's rules (axes aligned from
+// the last; an extent of 1 repeats).
+std::vector<std::size_t> broadcast_extents(std::initializer_list<const_array_ref> operands, const char *what)
+{
+	std::size_t rank = 0;
+	for (const auto &x : operands)
+	{
+		rank = std::max(rank, x->get_extents().size());
+	}
+	std::vector<std::size_t> out(rank, 1);
+	for (const auto &x : operands)
+	{
+		const auto &e = x->get_extents();
+		const std::size_t lead = rank - e.size();
+		for (std::size_t a = 0; a < e.size(); ++a)
+		{
+			std::size_t &o = out[lead + a];
+			if (e[a] == o || e[a] == 1)
+			{
+				continue;
+			}
+			if (o != 1)
+			{
+				throw std::invalid_argument(std::string(what) + ": operand extents do not broadcast");
+			}
+			o = e[a];
+		}
+	}
+	return out;
+}
+
+array run_elementwise(const operation &op, std::initializer_list<const_array_ref> inputs, numerical_type out_type,
+	const execution_context &context, array *out, const char *what)
+{
+	check_context(context);
+	array local;
+	array &target = out ? *out : local;
+	const array_descriptor planned(strided_layout::make_contiguous(broadcast_extents(inputs, what)), out_type);
+	const std::vector<const_array_ref> in(inputs);
+	context.get_dispatcher()->dispatch(op, {&target, 1}, {&planned, 1}, {in.data(), in.size()},
+		context.get_device_context());
+	return target;
+}
+
+} // namespace
+
+array divide(const_array_ref x, const_array_ref y, const execution_context &context, array *out)
+{
+	if (x->get_data_type() != y->get_data_type() || x->get_data_type() == numerical_type::boolean)
+	{
+		throw std::invalid_argument("divide: both operands take one non-boolean type");
+	}
+	return run_elementwise(ops::divide_operation(), {x, y}, x->get_data_type(), context, out, "divide");
+}
+
+array greater(const_array_ref x, const_array_ref y, const execution_context &context, array *out)
+{
+	if (x->get_data_type() != y->get_data_type())
+	{
+		throw std::invalid_argument("greater: both operands take one type");
+	}
+	return run_elementwise(ops::greater_operation(), {x, y}, numerical_type::boolean, context, out, "greater");
+}
+
+array where(const_array_ref condition, const_array_ref x, const_array_ref y, const execution_context &context,
+	array *out)
+{
+	if (condition->get_data_type() != numerical_type::boolean || x->get_data_type() != y->get_data_type())
+	{
+		throw std::invalid_argument("where: a boolean condition and two operands of one type");
+	}
+	return run_elementwise(ops::where_operation(), {condition, x, y}, x->get_data_type(), context, out, "where");
+}
+
+// ---------------------------------------------------------------------------
 // Fourier transforms
 // ---------------------------------------------------------------------------
 
@@ -860,6 +951,102 @@ void require_rank(const_array_ref x, std::size_t rank, const char *what)
 }
 
 } // namespace
+
+namespace
+{
+
+// Axes as absolute indices, every axis when empty; repeated axes refused.
+ops::axis_list absolute_axes(ops::axis_list axes, std::size_t rank, const char *what)
+{
+	if (axes.empty())
+	{
+		for (std::size_t a = 0; a < rank; ++a)
+		{
+			axes.push_back(static_cast<std::ptrdiff_t>(a));
+		}
+	}
+	for (auto &a : axes)
+	{
+		if (a < 0)
+		{
+			a += static_cast<std::ptrdiff_t>(rank);
+		}
+		if (a < 0 || a >= static_cast<std::ptrdiff_t>(rank))
+		{
+			throw std::invalid_argument(std::string(what) + ": an axis lies outside the operand");
+		}
+	}
+	for (std::size_t i = 0; i < axes.size(); ++i)
+	{
+		for (std::size_t j = i + 1; j < axes.size(); ++j)
+		{
+			if (axes[i] == axes[j])
+			{
+				throw std::invalid_argument(std::string(what) + ": an axis is repeated");
+			}
+		}
+	}
+	return axes;
+}
+
+numerical_type complex_result(numerical_type t, const char *what)
+{
+	switch (t)
+	{
+	case numerical_type::float32:
+	case numerical_type::complex_float32:
+		return numerical_type::complex_float32;
+	case numerical_type::float64:
+	case numerical_type::complex_float64:
+		return numerical_type::complex_float64;
+	default:
+		throw std::invalid_argument(std::string(what) + ": a floating-point operand is required, not " +
+			to_string(t));
+	}
+}
+
+} // namespace
+
+array fftn(const_array_ref x, ops::axis_list axes, const execution_context &context,
+	ops::fourier_normalization normalization, array *out)
+{
+	axes = absolute_axes(std::move(axes), x->get_extents().size(), "fftn");
+	return run_transform(ops::fft_operation(axes, normalization), x, x->get_extents(),
+		complex_result(x->get_data_type(), "fftn"), context, out);
+}
+
+array ifftn(const_array_ref x, ops::axis_list axes, const execution_context &context,
+	ops::fourier_normalization normalization, array *out)
+{
+	axes = absolute_axes(std::move(axes), x->get_extents().size(), "ifftn");
+	return run_transform(ops::ifft_operation(axes, normalization), x, x->get_extents(),
+		complex_result(x->get_data_type(), "ifftn"), context, out);
+}
+
+array rfftn(const_array_ref x, ops::axis_list axes, const execution_context &context,
+	ops::fourier_normalization normalization, array *out)
+{
+	axes = absolute_axes(std::move(axes), x->get_extents().size(), "rfftn");
+	std::vector<std::size_t> e = x->get_extents();
+	e[static_cast<std::size_t>(axes.back())] = e[static_cast<std::size_t>(axes.back())] / 2 + 1;
+	return run_transform(ops::rfft_operation(axes, normalization), x, e, complex_of(x->get_data_type()), context,
+		out);
+}
+
+array irfftn(const_array_ref x, ops::axis_list axes, std::size_t extent, const execution_context &context,
+	ops::fourier_normalization normalization, array *out)
+{
+	axes = absolute_axes(std::move(axes), x->get_extents().size(), "irfftn");
+	std::vector<std::size_t> e = x->get_extents();
+	const std::size_t last = static_cast<std::size_t>(axes.back());
+	if (e[last] != extent / 2 + 1)
+	{
+		throw std::invalid_argument("irfftn: the halved axis must hold extent / 2 + 1 coefficients");
+	}
+	e[last] = extent;
+	return run_transform(ops::irfft_operation(axes, extent, normalization), x, e, real_of(x->get_data_type()),
+		context, out);
+}
 
 array rfft2(const_array_ref x, ops::fourier_normalization normalization, const execution_context &context, array *out)
 {

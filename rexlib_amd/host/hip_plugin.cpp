@@ -10,9 +10,10 @@
 //     staging resource (pinned, hipHostMalloc), and command queues, each a
 //     HIP stream of default priority with its own event timeline;
 //   - program builders for insert_slices, insert_weighted_slices,
-//     insert_ctf_slices, rfft (last two axes), irfft (last two or three),
-//     fill (zeros) and copy, which run the kernels of librexlib_hip.so through
-//     its C ABI (include/rexlib_hip.h).
+//     insert_ctf_slices, fft, ifft, rfft and irfft (any axes; the square
+//     images and cubic grids of the path take dedicated kernels), fill, copy,
+//     divide, greater and where, which run the kernels of
+//     librexlib_hip.so through its C ABI (include/rexlib_hip.h).
 //
 // Queue semantics follow plugins/cuda/src/hardware/cuda_command_queue.cpp:102-215:
 // a command waits for its dependencies (an event wait for another HIP queue,
@@ -432,6 +433,8 @@ int abi_type(numerical_type t)
 		return REXLIB_HIP_COMPLEX128;
 	case numerical_type::float16:
 		return REXLIB_HIP_FLOAT16;
+	case numerical_type::boolean:
+		return REXLIB_HIP_BOOL;
 	default:
 		throw unsupported_operation_error(std::string("hip: no element type ") + to_string(t));
 	}
@@ -608,6 +611,23 @@ public:
 	}
 };
 
+// Lengths the fast square-image kernels take: prime factors 2, 3 and 5.
+bool smooth_length(std::size_t n)
+{
+	if (n == 0)
+	{
+		return false;
+	}
+	for (const std::size_t r : {2u, 3u, 5u})
+	{
+		while (n % r == 0)
+		{
+			n /= r;
+		}
+	}
+	return n == 1;
+}
+
 // Normalised axes of an operation on `rank` axes: the last `count` or nothing.
 bool last_axes(const ops::axis_list &axes, std::size_t rank, std::size_t count)
 {
@@ -625,6 +645,102 @@ bool last_axes(const ops::axis_list &axes, std::size_t rank, std::size_t count)
 	}
 	return true;
 }
+
+// Any other transform over any axes: the general programs of the C ABI.
+enum class transform_kind
+{
+	fft,
+	ifft,
+	rfft,
+	irfft
+};
+
+int32_t abi_normalization(ops::fourier_normalization n)
+{
+	switch (n)
+	{
+	case ops::fourier_normalization::ortho:
+		return 1;
+	case ops::fourier_normalization::forward:
+		return 2;
+	default:
+		return 0;
+	}
+}
+
+class general_transform_program final : public bound_program
+{
+public:
+	general_transform_program(span<const operand_descriptor> outputs, span<const operand_descriptor> inputs,
+		transform_kind kind, const ops::axis_list &axes, ops::fourier_normalization normalization)
+		: bound_program(outputs, inputs), m_kind(kind), m_normalization(abi_normalization(normalization))
+	{
+		const std::ptrdiff_t rank = static_cast<std::ptrdiff_t>(inputs[0].get_layout().get_rank());
+		for (const std::ptrdiff_t a : axes)
+		{
+			m_axes.push_back(static_cast<int32_t>(a < 0 ? a + rank : a));
+		}
+	}
+	void enqueue(span<const std::shared_ptr<buffer>> outputs, span<const std::shared_ptr<const buffer>> inputs,
+		hipStream_t stream) const override
+	{
+		std::vector<rexlib_hip_array> out;
+		const auto in = arrays(outputs, inputs, out);
+		const int32_t count = static_cast<int32_t>(m_axes.size());
+		switch (m_kind)
+		{
+		case transform_kind::fft:
+		case transform_kind::ifft:
+			abi_check(rexlib_hip_fft(&in[0], &out[0], m_axes.data(), count, m_kind == transform_kind::ifft ? 1 : 0,
+				m_normalization, stream));
+			break;
+		case transform_kind::rfft:
+			abi_check(rexlib_hip_rfft(&in[0], &out[0], m_axes.data(), count, m_normalization, stream));
+			break;
+		case transform_kind::irfft:
+			abi_check(rexlib_hip_irfft(&in[0], &out[0], m_axes.data(), count, m_normalization, stream));
+			break;
+		}
+	}
+
+private:
+	transform_kind m_kind;
+	int32_t m_normalization;
+	std::vector<int32_t> m_axes;
+};
+
+bool general_transform_suitable(span<const operand_descriptor> outputs, span<const operand_descriptor> inputs,
+	command_queue &queue, numerical_type in_type, numerical_type out_type, const ops::axis_list &axes)
+{
+	return on_hip_queue(queue) && outputs.size() == 1 && inputs.size() == 1 && all_device_accessible(outputs) &&
+		all_device_accessible(inputs) && inputs[0].get_data_type() == in_type &&
+		outputs[0].get_data_type() == out_type && !axes.empty() &&
+		inputs[0].get_layout().get_rank() <= REXLIB_HIP_MAX_RANK &&
+		inputs[0].get_layout().get_rank() == outputs[0].get_layout().get_rank();
+}
+
+template <typename Operation, transform_kind Kind>
+class complex_transform_builder final : public program_builder
+{
+public:
+	operation_id get_operation_id() const noexcept override { return operation_id::of<Operation>(); }
+	backend_priority get_suitability(const operation &op, span<const operand_descriptor> outputs,
+		span<const operand_descriptor> inputs, command_queue &queue) const override
+	{
+		const auto &f = operation_cast<Operation>(op);
+		return general_transform_suitable(outputs, inputs, queue, numerical_type::complex_float32,
+				   numerical_type::complex_float32, f.get_axes())
+			? backend_priority::optimal
+			: backend_priority::unsupported;
+	}
+	std::shared_ptr<program> build(const operation &op, span<const operand_descriptor> outputs,
+		span<const operand_descriptor> inputs, command_queue &, program_cache *) const override
+	{
+		const auto &f = operation_cast<Operation>(op);
+		return std::make_shared<general_transform_program>(outputs, inputs, Kind, f.get_axes(),
+			f.get_normalization());
+	}
+};
 
 class rfft_program final : public bound_program
 {
@@ -647,6 +763,31 @@ public:
 		span<const operand_descriptor> inputs, command_queue &queue) const override
 	{
 		const auto &f = operation_cast<ops::rfft_operation>(op);
+		if (fast(f, outputs, inputs, queue) ||
+			general_transform_suitable(outputs, inputs, queue, numerical_type::float32, numerical_type::complex_float32,
+				f.get_axes()))
+		{
+			return backend_priority::optimal;
+		}
+		return backend_priority::unsupported;
+	}
+	std::shared_ptr<program> build(const operation &op, span<const operand_descriptor> outputs,
+		span<const operand_descriptor> inputs, command_queue &queue, program_cache *) const override
+	{
+		const auto &f = operation_cast<ops::rfft_operation>(op);
+		if (fast(f, outputs, inputs, queue))
+		{
+			return std::make_shared<rfft_program>(outputs, inputs);
+		}
+		return std::make_shared<general_transform_program>(outputs, inputs, transform_kind::rfft, f.get_axes(),
+			f.get_normalization());
+	}
+
+private:
+	// rfft2 of square images: the four-step and packed-row kernels.
+	static bool fast(const ops::rfft_operation &f, span<const operand_descriptor> outputs,
+		span<const operand_descriptor> inputs, command_queue &queue)
+	{
 		if (!on_hip_queue(queue) || outputs.size() != 1 || inputs.size() != 1 || !all_device_accessible(outputs) ||
 			!all_device_accessible(inputs) || f.get_normalization() != ops::fourier_normalization::backward ||
 			!last_axes(f.get_axes(), inputs[0].get_layout().get_rank(), 2) ||
@@ -654,14 +795,10 @@ public:
 			(inputs[0].get_data_type() != numerical_type::float32 &&
 				inputs[0].get_data_type() != numerical_type::float16))
 		{
-			return backend_priority::unsupported;
+			return false;
 		}
-		return backend_priority::optimal;
-	}
-	std::shared_ptr<program> build(const operation &, span<const operand_descriptor> outputs,
-		span<const operand_descriptor> inputs, command_queue &, program_cache *) const override
-	{
-		return std::make_shared<rfft_program>(outputs, inputs);
+		const auto &e = inputs[0].get_layout().get_extents();
+		return e[e.size() - 1] == e[e.size() - 2] && smooth_length(e.back());
 	}
 };
 
@@ -693,64 +830,180 @@ public:
 		span<const operand_descriptor> inputs, command_queue &queue) const override
 	{
 		const auto &f = operation_cast<ops::irfft_operation>(op);
+		if (fast(f, outputs, inputs, queue) ||
+			general_transform_suitable(outputs, inputs, queue, numerical_type::complex_float32, numerical_type::float32,
+				f.get_axes()))
+		{
+			return backend_priority::optimal;
+		}
+		return backend_priority::unsupported;
+	}
+	std::shared_ptr<program> build(const operation &op, span<const operand_descriptor> outputs,
+		span<const operand_descriptor> inputs, command_queue &queue, program_cache *) const override
+	{
+		const auto &f = operation_cast<ops::irfft_operation>(op);
+		if (fast(f, outputs, inputs, queue))
+		{
+			return std::make_shared<irfft_program>(outputs, inputs, static_cast<int>(f.get_axes().size()));
+		}
+		return std::make_shared<general_transform_program>(outputs, inputs, transform_kind::irfft, f.get_axes(),
+			f.get_normalization());
+	}
+
+private:
+	// irfft2 / irfft3 of square or cubic grids of a smooth extent.
+	static bool fast(const ops::irfft_operation &f, span<const operand_descriptor> outputs,
+		span<const operand_descriptor> inputs, command_queue &queue)
+	{
 		if (!on_hip_queue(queue) || outputs.size() != 1 || inputs.size() != 1 || !all_device_accessible(outputs) ||
 			!all_device_accessible(inputs) || f.get_normalization() != ops::fourier_normalization::backward ||
 			outputs[0].get_data_type() != numerical_type::float32 ||
 			inputs[0].get_data_type() != numerical_type::complex_float32)
 		{
-			return backend_priority::unsupported;
+			return false;
 		}
 		const std::size_t rank = inputs[0].get_layout().get_rank();
-		if (!last_axes(f.get_axes(), rank, 2) && !last_axes(f.get_axes(), rank, 3))
+		const std::size_t dims = last_axes(f.get_axes(), rank, 2) ? 2 : (last_axes(f.get_axes(), rank, 3) ? 3 : 0);
+		if (dims == 0)
 		{
-			return backend_priority::unsupported;
+			return false;
 		}
-		return backend_priority::optimal;
-	}
-	std::shared_ptr<program> build(const operation &op, span<const operand_descriptor> outputs,
-		span<const operand_descriptor> inputs, command_queue &, program_cache *) const override
-	{
-		const auto &f = operation_cast<ops::irfft_operation>(op);
-		return std::make_shared<irfft_program>(outputs, inputs, static_cast<int>(f.get_axes().size()));
+		const auto &e = outputs[0].get_layout().get_extents();
+		for (std::size_t a = rank - dims; a < rank; ++a)
+		{
+			if (e[a] != e.back())
+			{
+				return false;
+			}
+		}
+		return smooth_length(e.back());
 	}
 };
 
-// Zero fill of a contiguous operand in device memory.
+// Fill of an operand in device memory with the operation's value, by a fill
+// kernel (not hipMemsetAsync, so that a captured graph holds kernel nodes
+// only), any strides.
 class fill_program final : public bound_program
 {
 public:
-	using bound_program::bound_program;
-	void enqueue(span<const std::shared_ptr<buffer>> outputs, span<const std::shared_ptr<const buffer>>,
+	fill_program(span<const operand_descriptor> outputs, span<const operand_descriptor> inputs, double value)
+		: bound_program(outputs, inputs), m_value(value)
+	{
+	}
+	void enqueue(span<const std::shared_ptr<buffer>> outputs, span<const std::shared_ptr<const buffer>> inputs,
 		hipStream_t stream) const override
 	{
-		const operand_descriptor &d = m_outputs[0];
-		char *base = static_cast<char *>(operand_address(*outputs[0])) +
-			d.get_layout().get_offset() * static_cast<std::ptrdiff_t>(get_size(d.get_data_type()));
-		hip_check(hipMemsetAsync(base, 0, d.get_layout().get_element_count() * get_size(d.get_data_type()), stream),
-			"hipMemsetAsync");
+		std::vector<rexlib_hip_array> out;
+		arrays(outputs, inputs, out);
+		abi_check(rexlib_hip_fill(&out[0], m_value, 0.0, stream));
 	}
+
+private:
+	double m_value;
 };
+
+bool fill_type(numerical_type t)
+{
+	return t == numerical_type::float32 || t == numerical_type::float64 || t == numerical_type::complex_float32 ||
+		t == numerical_type::complex_float64 || t == numerical_type::float16 || t == numerical_type::boolean;
+}
 
 class fill_builder final : public program_builder
 {
 public:
 	operation_id get_operation_id() const noexcept override { return operation_id::of<ops::fill_operation>(); }
-	backend_priority get_suitability(const operation &op, span<const operand_descriptor> outputs,
+	backend_priority get_suitability(const operation &, span<const operand_descriptor> outputs,
 		span<const operand_descriptor> inputs, command_queue &queue) const override
 	{
-		const auto &f = operation_cast<ops::fill_operation>(op);
 		if (!on_hip_queue(queue) || outputs.size() != 1 || !inputs.empty() || !all_device_accessible(outputs) ||
-			!outputs[0].get_layout().is_contiguous() || f.get_value() != 0.0 ||
-			get_size(outputs[0].get_data_type()) == 0)
+			!fill_type(outputs[0].get_data_type()) || outputs[0].get_layout().get_rank() > REXLIB_HIP_MAX_RANK)
 		{
 			return backend_priority::unsupported;
 		}
 		return backend_priority::normal;
 	}
+	std::shared_ptr<program> build(const operation &op, span<const operand_descriptor> outputs,
+		span<const operand_descriptor> inputs, command_queue &, program_cache *) const override
+	{
+		return std::make_shared<fill_program>(outputs, inputs, operation_cast<ops::fill_operation>(op).get_value());
+	}
+};
+
+// divide, greater and where through the elementwise programs of the C ABI:
+// float32 and complex64 operands (float32 for greater), boolean conditions
+// and comparisons, broadcasting, any strides.
+enum class elementwise_kind
+{
+	divide,
+	greater,
+	where
+};
+
+class elementwise_program final : public bound_program
+{
+public:
+	elementwise_program(span<const operand_descriptor> outputs, span<const operand_descriptor> inputs,
+		elementwise_kind kind)
+		: bound_program(outputs, inputs), m_kind(kind)
+	{
+	}
+	void enqueue(span<const std::shared_ptr<buffer>> outputs, span<const std::shared_ptr<const buffer>> inputs,
+		hipStream_t stream) const override
+	{
+		std::vector<rexlib_hip_array> out;
+		const auto in = arrays(outputs, inputs, out);
+		switch (m_kind)
+		{
+		case elementwise_kind::divide:
+			abi_check(rexlib_hip_divide(&in[0], &in[1], &out[0], stream));
+			break;
+		case elementwise_kind::greater:
+			abi_check(rexlib_hip_greater(&in[0], &in[1], &out[0], stream));
+			break;
+		case elementwise_kind::where:
+			abi_check(rexlib_hip_where(&in[0], &in[1], &in[2], &out[0], stream));
+			break;
+		}
+	}
+
+private:
+	elementwise_kind m_kind;
+};
+
+template <typename Operation, elementwise_kind Kind>
+class elementwise_builder final : public program_builder
+{
+public:
+	operation_id get_operation_id() const noexcept override { return operation_id::of<Operation>(); }
+	backend_priority get_suitability(const operation &, span<const operand_descriptor> outputs,
+		span<const operand_descriptor> inputs, command_queue &queue) const override
+	{
+		const std::size_t arity = Kind == elementwise_kind::where ? 3 : 2;
+		if (!on_hip_queue(queue) || outputs.size() != 1 || inputs.size() != arity || !all_device_accessible(outputs) ||
+			!all_device_accessible(inputs))
+		{
+			return backend_priority::unsupported;
+		}
+		for (const auto &d : inputs)
+		{
+			if (d.get_layout().get_rank() > REXLIB_HIP_MAX_RANK)
+			{
+				return backend_priority::unsupported;
+			}
+		}
+		const numerical_type value = inputs[Kind == elementwise_kind::where ? 1 : 0].get_data_type();
+		const bool real = value == numerical_type::float32;
+		const bool ok = Kind == elementwise_kind::greater
+			? real && inputs[1].get_data_type() == value && outputs[0].get_data_type() == numerical_type::boolean
+			: (real || value == numerical_type::complex_float32) && outputs[0].get_data_type() == value &&
+				inputs[arity - 1].get_data_type() == value &&
+				(Kind != elementwise_kind::where || inputs[0].get_data_type() == numerical_type::boolean);
+		return ok ? backend_priority::optimal : backend_priority::unsupported;
+	}
 	std::shared_ptr<program> build(const operation &, span<const operand_descriptor> outputs,
 		span<const operand_descriptor> inputs, command_queue &, program_cache *) const override
 	{
-		return std::make_shared<fill_program>(outputs, inputs);
+		return std::make_shared<elementwise_program>(outputs, inputs, Kind);
 	}
 };
 
@@ -819,9 +1072,19 @@ public:
 		programs->register_builder(std::make_unique<insertion_builder<em::insert_weighted_slices_operation>>());
 		programs->register_builder(std::make_unique<insertion_builder<em::insert_ctf_slices_operation>>());
 		programs->register_builder(std::make_unique<rfft_builder>());
+		programs->register_builder(
+			std::make_unique<complex_transform_builder<ops::fft_operation, transform_kind::fft>>());
+		programs->register_builder(
+			std::make_unique<complex_transform_builder<ops::ifft_operation, transform_kind::ifft>>());
 		programs->register_builder(std::make_unique<irfft_builder>());
 		programs->register_builder(std::make_unique<fill_builder>());
 		programs->register_builder(std::make_unique<copy_builder>());
+		programs->register_builder(
+			std::make_unique<elementwise_builder<ops::divide_operation, elementwise_kind::divide>>());
+		programs->register_builder(
+			std::make_unique<elementwise_builder<ops::greater_operation, elementwise_kind::greater>>());
+		programs->register_builder(
+			std::make_unique<elementwise_builder<ops::where_operation, elementwise_kind::where>>());
 	}
 };
 

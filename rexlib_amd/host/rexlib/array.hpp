@@ -152,6 +152,28 @@ public:
 	const char *get_name() const noexcept override { return "copy"; }
 };
 
+// result = dividend / divisor, elementwise with broadcasting, one element
+// type (include/rexlib/ops/arithmetic/divide_operation.hpp).
+class REXLIB_API divide_operation final : public basic_operation<divide_operation>
+{
+public:
+	const char *get_name() const noexcept override { return "divide"; }
+};
+
+// result = x > y, boolean (include/rexlib/ops/compare/greater_operation.hpp).
+class REXLIB_API greater_operation final : public basic_operation<greater_operation>
+{
+public:
+	const char *get_name() const noexcept override { return "greater"; }
+};
+
+// result = condition ? x : y (include/rexlib/ops/selection/where_operation.hpp).
+class REXLIB_API where_operation final : public basic_operation<where_operation>
+{
+public:
+	const char *get_name() const noexcept override { return "where"; }
+};
+
 } // namespace ops
 
 // Host arrays, in the core's host memory.
@@ -160,5 +182,15 @@ REXLIB_API array zeros(std::vector<std::size_t> extents, numerical_type type, co
 REXLIB_API array to_device(const_array_ref x, const execution_context &context, array *out = nullptr);
 // Copies to a new host array and waits for the copy.
 REXLIB_API array to_host(const_array_ref x, const execution_context &context);
+
+// Elementwise functions (include/rexlib/functional/arithmetic.hpp:125-130,
+// compare.hpp:101-106, selection.hpp:27-33, creation.hpp): operands broadcast
+// against each other; the result takes the broadcast extents.
+REXLIB_API array full(std::vector<std::size_t> extents, double value, numerical_type type,
+	const execution_context &context);
+REXLIB_API array divide(const_array_ref x, const_array_ref y, const execution_context &context, array *out = nullptr);
+REXLIB_API array greater(const_array_ref x, const_array_ref y, const execution_context &context, array *out = nullptr);
+REXLIB_API array where(const_array_ref condition, const_array_ref x, const_array_ref y,
+	const execution_context &context, array *out = nullptr);
 
 } // namespace rexlib

@@ -1,10 +1,11 @@
 """Autotuner of the insertion's brick schedule.
 
 For a workload (slice extent n, volume extent N, slices per launch L) it times
-the insertion on the device for every brick shape the library holds (log2 of
-the brick's extents along x, y and z) and several task sizes (records per
-task), checks every configuration's sums against the first one's, and records
-the fastest per volume extent N in a JSON table. ``apply_all()`` hands the
+the insertion on the device for every brick shape the library holds (the
+brick's extents along y and z; 16 along x) and several shares (expected plane
+hits per task, which split busy bricks), checks every configuration's sums
+against the first one's, and records the fastest per volume extent N in a
+JSON table. ``apply_all()`` hands the
 whole table to the library, which then picks each call's entry by its volume
 extent, so reconstructions of different sizes need no switching between
 calls.
@@ -24,10 +25,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
-# (log2 x, log2 y, log2 z) of the bricks the library is built for.
-SHAPES = ((3, 3, 3), (4, 3, 3), (3, 3, 2), (4, 3, 2))
-RECORDS_PER_TASK = (2048, 4096, 8192, 16384, 32768)
-DEFAULT = {"shift": [3, 3, 3], "records_per_task": 8192}
+# (y, z) extents of the bricks the library is built for.
+SHAPES = ((8, 6), (8, 8), (8, 16), (16, 8), (16, 16))
+SHARES = (64, 96, 128, 192, 256, 512)
+DEFAULT = {"extent_y": 8, "extent_z": 6, "share": 256}
 
 
 def load_table(path: str = TABLE) -> dict:
@@ -37,18 +38,18 @@ def load_table(path: str = TABLE) -> dict:
         return json.load(f)
 
 
-def set_tuning(extent: int, shift, records_per_task: int) -> None:
+def set_tuning(extent: int, shape, share: int) -> None:
     """The library's entry for volumes of extent ``extent`` (0: every other extent)."""
     from . import _native as N
-    N.check(N.library().rexlib_hip_set_insertion_tuning(int(extent), int(shift[0]), int(shift[1]), int(shift[2]),
-                                                         int(records_per_task)))
+    N.check(N.library().rexlib_hip_set_insertion_tuning(int(extent), int(shape[0]), int(shape[1]), int(share)))
 
 
 def get_tuning(extent: int):
+    """((extent_y, extent_z), share) that calls with volume extent ``extent`` use."""
     from . import _native as N
-    v = [ctypes.c_int32() for _ in range(4)]
+    v = [ctypes.c_int32() for _ in range(3)]
     N.check(N.library().rexlib_hip_get_insertion_tuning(int(extent), *[ctypes.byref(x) for x in v]))
-    return (v[0].value, v[1].value, v[2].value), v[3].value
+    return (v[0].value, v[1].value), v[2].value
 
 
 def clear_tuning() -> None:
@@ -60,7 +61,7 @@ def apply_all(path: str = TABLE) -> dict:
     """Hand every recorded entry (keyed by N) to the library; returns them."""
     table = load_table(path)
     for key, entry in table.items():
-        set_tuning(int(key), entry["shift"], entry["records_per_task"])
+        set_tuning(int(key), (entry["extent_y"], entry["extent_z"]), entry["share"])
     return table
 
 
@@ -120,30 +121,31 @@ def tune(n: int, big: int, launch: int, path: str = TABLE, verbose: bool = True)
     reference = None
     previous = get_tuning(big)
     try:
-        for shift in SHAPES:
-            for per_task in RECORDS_PER_TASK:
-                set_tuning(big, shift, per_task)
+        for shape in SHAPES:
+            for share in SHARES:
+                set_tuning(big, shape, share)
                 ms, sums = time_insertion(n, big, slices, transforms, ctf)
                 if reference is None:
                     reference = sums
                 error = max(rel_l2(sums[0], reference[0]), rel_l2(sums[1], reference[1]))
                 if not error <= 1e-5:
-                    raise RuntimeError(f"brick shape {shift}, {per_task} records per task: sums differ from "
+                    raise RuntimeError(f"brick shape {shape}, share {share}: sums differ from "
                                        f"the first configuration's by rel-L2 {error:.3e}")
-                results.append({"shift": list(shift), "records_per_task": per_task, "ms": ms, "rel_l2": error})
+                results.append({"extent_y": shape[0], "extent_z": shape[1], "share": share, "ms": ms,
+                                "rel_l2": error})
                 if verbose:
-                    print(f"n={n} N={big} L={launch} shift={shift} records/task={per_task}: {ms:.3f} ms "
+                    print(f"n={n} N={big} L={launch} brick 16x{shape[0]}x{shape[1]} share={share}: {ms:.3f} ms "
                           f"(rel-L2 to the first {error:.1e})", flush=True)
     finally:
         set_tuning(big, *previous)
     best = min(results, key=lambda r: r["ms"])
     table = load_table(path)
-    table[str(big)] = {"shift": best["shift"], "records_per_task": best["records_per_task"], "n": n,
+    table[str(big)] = {"extent_y": best["extent_y"], "extent_z": best["extent_z"], "share": best["share"], "n": n,
                        "launch": launch, "ms": best["ms"], "images_per_s": launch / (best["ms"] * 1e-3),
                        "tuned": time.strftime("%Y-%m-%d")}
     with open(path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
-    set_tuning(big, best["shift"], best["records_per_task"])
+    set_tuning(big, (best["extent_y"], best["extent_z"]), best["share"])
     return {"best": best, "all": results}
 
 

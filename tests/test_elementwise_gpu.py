@@ -1,0 +1,75 @@
+"""The elementwise programs of the quotient (fill, divide, greater, where) on
+the GPU against numpy, with broadcasting and strided views, and the
+reference test recipe's quotient written with them
+(tests/integration/src/em/functional/test_cpu_reconstruction.cpp:457-464)
+against em.normalize and the oracle."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from gpu_helpers import to_device
+from rexlib_amd import em, functional as F
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dtype,value", [(torch.float32, 2.5), (torch.float64, -1.25), (torch.complex64, 1 - 2j),
+                                         (torch.complex128, 3j), (torch.float16, 0.1), (torch.bfloat16, 0.1),
+                                         (torch.bool, True)])
+def test_fill_every_type(dtype, value):
+    out = torch.empty((3, 5, 7), dtype=dtype, device="cuda")
+    F.fill_(out, value)
+    expected = torch.full((3, 5, 7), value, dtype=dtype)
+    assert torch.equal(out.cpu(), expected)
+    view = torch.zeros((6, 10), dtype=dtype, device="cuda")[::2, 1::3]
+    F.fill_(view, value)
+    assert torch.equal(view.cpu(), torch.full(view.shape, value, dtype=dtype))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.complex64])
+def test_divide_matches_numpy_with_broadcasting(dtype):
+    rng = np.random.default_rng(1)
+    a = rng.normal(size=(4, 6, 5)).astype(dtype)
+    b = rng.uniform(0.5, 2.0, size=(6, 1)).astype(dtype)
+    if np.iscomplexobj(a):
+        a = a + 1j * rng.normal(size=a.shape).astype(np.float32)
+        b = b + 1j * rng.uniform(0.5, 2.0, size=b.shape).astype(np.float32)
+    got = F.divide(to_device(a), to_device(b)).cpu().numpy()
+    want = a.astype(np.complex128 if np.iscomplexobj(a) else np.float64) / b
+    assert got.shape == want.shape
+    np.testing.assert_allclose(got, want, rtol=2e-6, atol=0)
+
+
+def test_complex_division_by_large_and_small_parts():
+    # Smith's scaling keeps the quotient finite where c^2 + d^2 overflows float32.
+    a = torch.tensor([1e30 + 1e30j, 1 + 1j], dtype=torch.complex64, device="cuda")
+    b = torch.tensor([3e30 + 4e30j, 1e-30 + 0j], dtype=torch.complex64, device="cuda")
+    got = F.divide(a, b).cpu().numpy().astype(np.complex128)
+    want = a.cpu().numpy().astype(np.complex128) / b.cpu().numpy().astype(np.complex128)
+    np.testing.assert_allclose(got, want, rtol=1e-6)
+
+
+def test_greater_and_where_on_strided_views():
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(8, 9)).astype(np.float32)
+    t = to_device(x)[:, ::2]
+    mask = F.greater(t, 0.25)
+    assert mask.dtype == torch.bool
+    assert np.array_equal(mask.cpu().numpy(), x[:, ::2] > 0.25)
+    picked = F.where(mask, t, -1.0).cpu().numpy()
+    assert np.array_equal(picked, np.where(x[:, ::2] > 0.25, x[:, ::2], np.float32(-1.0)))
+
+
+def test_quotient_recipe_equals_normalize_and_the_oracle():
+    rng = np.random.default_rng(3)
+    shape = (16, 16, 9)
+    num = (rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64)
+    den = rng.uniform(-0.01, 0.05, shape).astype(np.float32)
+    n_dev, d_dev = to_device(num), to_device(den)
+    den_c = torch.empty(shape, dtype=torch.complex64, device="cuda")
+    F.where(F.greater(d_dev, 0.0), d_dev.to(torch.complex64), 1.0, out=den_c)
+    q = F.where(F.greater(d_dev, 1e-3), F.divide(n_dev, den_c), 0.0)
+    ref = oracle.normalize(num.astype(np.complex128), den.astype(np.float64), 1e-3)
+    assert np.max(np.abs(q.cpu().numpy() - ref)) <= 1e-6 * np.max(np.abs(ref))
+    assert np.array_equal(q.cpu().numpy(), em.normalize(n_dev, d_dev, 1e-3).cpu().numpy())

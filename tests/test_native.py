@@ -181,3 +181,45 @@ def test_normalize_and_shell_sums_check_their_operands():
     assert lib.rexlib_hip_shell_sums(ctypes.byref(num), ctypes.byref(b), None, None) == N.UNSUPPORTED
     batched = fake((2, 8, 8, 5), N.COMPLEX64)
     assert lib.rexlib_hip_shell_sums(ctypes.byref(batched), ctypes.byref(batched), None, None) == N.UNSUPPORTED
+
+
+def test_elementwise_programs_check_their_operands():
+    lib = N.library()
+    a = fake((4, 5), N.FLOAT32)
+    b = fake((3, 5), N.FLOAT32, 0x2000)
+    out = fake((4, 5), N.FLOAT32, 0x3000)
+    # (3, 5) does not broadcast to (4, 5).
+    assert lib.rexlib_hip_divide(ctypes.byref(a), ctypes.byref(b), ctypes.byref(out), None) == N.INVALID_ARGUMENT
+    assert "broadcast" in message()
+    c = fake((4, 5), N.COMPLEX64, 0x2000)
+    assert lib.rexlib_hip_divide(ctypes.byref(a), ctypes.byref(c), ctypes.byref(out), None) == N.INVALID_ARGUMENT
+    mask = fake((4, 5), N.BOOL, 0x4000)
+    assert lib.rexlib_hip_greater(ctypes.byref(a), ctypes.byref(a), ctypes.byref(out), None) == N.INVALID_ARGUMENT
+    assert lib.rexlib_hip_where(ctypes.byref(a), ctypes.byref(a), ctypes.byref(a), ctypes.byref(out), None) == \
+        N.INVALID_ARGUMENT
+    d = fake((4, 5), N.FLOAT64, 0x2000)
+    assert lib.rexlib_hip_greater(ctypes.byref(d), ctypes.byref(d), ctypes.byref(mask), None) == N.UNSUPPORTED
+
+
+def test_insertion_tuning_is_a_table_per_volume_extent():
+    """The brick tuning is kept per volume extent (0: every other extent),
+    checked on entry, and read whole by each call; no device is needed."""
+    from rexlib_amd import tune
+    lib = N.library()
+    tune.clear_tuning()
+    try:
+        assert tune.get_tuning(256) == ((8, 6), 256)
+        tune.set_tuning(256, (16, 8), 128)
+        assert tune.get_tuning(256) == ((16, 8), 128)
+        assert tune.get_tuning(64) == ((8, 6), 256)
+        tune.set_tuning(0, (8, 8), 64)
+        assert tune.get_tuning(64) == ((8, 8), 64)
+        assert tune.get_tuning(256) == ((16, 8), 128)
+        assert lib.rexlib_hip_set_insertion_tuning(96, 8, 7, 128) == N.INVALID_ARGUMENT
+        assert lib.rexlib_hip_set_insertion_tuning(96, 16, 6, 128) == N.INVALID_ARGUMENT
+        assert lib.rexlib_hip_set_insertion_tuning(96, 8, 6, 8) == N.INVALID_ARGUMENT
+        assert lib.rexlib_hip_set_insertion_tuning(-1, 8, 6, 64) == N.INVALID_ARGUMENT
+        assert tune.get_tuning(96) == ((8, 8), 64)
+    finally:
+        tune.clear_tuning()
+    assert tune.get_tuning(256) == ((8, 6), 256)

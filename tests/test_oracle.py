@@ -278,3 +278,35 @@ def test_normalize_threshold():
     num = np.array([1 + 1j, 2, 3], np.complex128)
     den = np.array([2.0, 1e-4, 0.5])
     np.testing.assert_allclose(oracle.normalize(num, den), [0.5 + 0.5j, 0, 6])
+
+
+def test_shell_is_round_half_up_of_the_radius_by_the_integer_walk():
+    # The reference's walk (row_insertion_kernel.inl:180-220): the largest s
+    # with 4 s^2 - 4 s + 1 <= 4 |k|^2; equal to floor(|k| + 1/2), ties up.
+    for kx in range(0, 300, 7):
+        for ky in range(-300, 300, 11):
+            q4 = 4 * (kx * kx + ky * ky)
+            s = 0
+            while 4 * (s + 1) ** 2 - 4 * (s + 1) + 1 <= q4:
+                s += 1
+            assert s == math.floor(math.sqrt(kx * kx + ky * ky) + 0.5)
+    # Exact half-integers round up: |k|^2 = (s + 1/2)^2 has no integer
+    # solution, so ties never arise on the lattice; 4|k|^2 = (2s+1)^2 - 1 is
+    # the last point of shell s.
+    for s in range(1, 200):
+        q4 = (2 * s + 1) ** 2 - 1
+        assert 4 * s * s - 4 * s + 1 <= q4 < 4 * (s + 1) ** 2 - 4 * (s + 1) + 1
+
+
+def test_one_select_each_way_makes_the_float_shell_estimate_exact():
+    # The device's shell: s = int(sqrtf(q) + 0.5f) in float32, then one step
+    # up and one down by the walk's own tests. Dense q up to 4e6 and random q
+    # up to 2^29 (|k|^2 of slices up to 16384 points).
+    rng = np.random.default_rng(0)
+    q = np.concatenate([np.arange(0, 4_000_000, dtype=np.int64), rng.integers(0, 2 ** 29, 2_000_000)])
+    s = (np.sqrt(q.astype(np.float32)) + np.float32(0.5)).astype(np.int64)
+    q4 = 4 * q
+    s = s + (4 * s * s + 4 * s + 1 <= q4)
+    s = s - ((s > 0) & (4 * s * s - 4 * s + 1 > q4))
+    exact = np.floor(np.sqrt(q.astype(np.float64)) + 0.5).astype(np.int64)
+    assert np.array_equal(s, exact)

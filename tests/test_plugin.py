@@ -14,7 +14,7 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-PLUGIN = os.path.join(ROOT, "rexlib_amd", "librexlib-hip.so")
+PLUGIN = os.path.join(ROOT, "rexlib_amd", "rexlib-plugins", "librexlib-hip.so")
 DRIVER = os.path.join(ROOT, "tests", "plugin_driver")
 HEADER = os.path.join(ROOT, "include", "rexlib_hip_plugin.h")
 
@@ -71,8 +71,10 @@ def test_registration_and_rules():
     assert plugin["devices"] >= 0
     assert plugin["operations"] == [
         "rexlib::em::insert_slices_operation", "rexlib::em::insert_weighted_slices_operation",
-        "rexlib::em::insert_ctf_slices_operation", "rexlib::ops::rfft_operation", "rexlib::ops::irfft_operation",
-        "rexlib::ops::fill_operation", "rexlib::ops::copy_operation"]
+        "rexlib::em::insert_ctf_slices_operation", "rexlib::ops::rfft_operation", "rexlib::ops::fft_operation",
+        "rexlib::ops::ifft_operation", "rexlib::ops::irfft_operation", "rexlib::ops::fill_operation",
+        "rexlib::ops::copy_operation", "rexlib::ops::divide_operation", "rexlib::ops::greater_operation",
+        "rexlib::ops::where_operation"]
     assert errors["foreign_queue"] == "unsupported_operation_error"
     assert errors["bad_pixel_size"] == errors["bad_optics"] == errors["bad_window"] == "invalid_argument"
     # lambda(300 kV), tests/unitary/src/em/ops/parameters/test_ctf_optics.cpp:40-53.

@@ -1,7 +1,8 @@
 """The plugin boundary on the GPU: a C++ caller of the mirrored rexlib API
 (tests/plugin_driver.cpp) loads librexlib-hip.so, opens hip:0, and runs
-rfft2, insert_ctf_slices in two calls that accumulate in place, and irfft3,
-with no HIP code of its own. Results are checked against numpy's pocketfft
+rfft2, insert_ctf_slices in two calls that accumulate in place, irfft3, and
+the quotient as where(greater(den, 1e-3), divide(num, den), 0), with no HIP
+code of its own. Results are checked against numpy's pocketfft
 and the double-precision oracle, within the reference's float32 tolerance
 (rel-L2 1e-4, tests/unitary/src/backends/cpu/builders/em/reconstruction/
 test_slice_insertion_program_builder.cpp:190). Errors cross the boundary as
@@ -54,6 +55,10 @@ def test_reconstruction_through_the_plugin(tmp_path, profile):
                                     ctf.astype(np.float64), radial.astype(np.float64)], w, pixel_size=1.5)
     assert oracle.rel_l2(num, ref[0]) <= 1e-4
     assert oracle.rel_l2(den, ref[1]) <= 1e-4
+
+    quotient = np.fromfile(tmp_path / "quotient_out.bin", np.complex64).reshape(big, big, hb)
+    want_q = oracle.normalize(num.astype(np.complex128), den.astype(np.float64), 1e-3)
+    assert np.max(np.abs(quotient - want_q)) <= 1e-6 * np.max(np.abs(want_q))
 
     volume = np.fromfile(tmp_path / "volume_out.bin", np.float32).reshape(big, big, big)
     assert oracle.rel_l2(volume, np.fft.irfftn(num.astype(np.complex128), s=(big, big, big), axes=(0, 1, 2))) < 1e-4
