@@ -197,6 +197,33 @@ int rexlib_hip_insert_ctf_slices(
 	void *stream);
 
 /*
+ * rexlib_hip_insert_ctf_slices that allocates nothing. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same operands and results as rexlib_hip_insert_ctf_slices; its
+ * workspace (slice entries, phase tables, the brick task list, prepared
+ * samples) lies in one fixed 128 MiB device array of the library, one per
+ * GPU, held until the process ends. A call whose entries and tables (3.2 KB
+ * per 256^2 slice) with one slice's prepared samples do not fit returns
+ * REXLIB_HIP_INVALID_ARGUMENT; larger calls go through in chunks of slices.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other streams, other threads) race.
+ */
+int rexlib_hip_insert_ctf_slices_no_alloc(
+	const rexlib_hip_array *numerator,
+	const rexlib_hip_array *denominator,
+	const rexlib_hip_array *slices,
+	const rexlib_hip_array *transforms,
+	const rexlib_hip_array *ctf,
+	const rexlib_hip_array *radial_weights,
+	const rexlib_hip_window *window,
+	const rexlib_hip_ctf_optics *optics,
+	double pixel_size,
+	const rexlib_hip_array *numerator_out,
+	const rexlib_hip_array *denominator_out,
+	void *stream);
+
+/*
  * Fused form of rfft2 followed by insert_ctf_slices for real images
  * (..., n, n) stored as float32, bfloat16, or float16 (float16 optionally with
  * a per-image scale (..., ) float32 that multiplies each image). The images are

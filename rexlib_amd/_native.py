@@ -62,6 +62,7 @@ EXPORTS = [
     "rexlib_hip_insert_slices",
     "rexlib_hip_insert_weighted_slices",
     "rexlib_hip_insert_ctf_slices",
+    "rexlib_hip_insert_ctf_slices_no_alloc",
     "rexlib_hip_insert_ctf_images",
     "rexlib_hip_rfft2",
     "rexlib_hip_irfft2",
@@ -117,6 +118,7 @@ def library() -> ctypes.CDLL:
     lib.rexlib_hip_insert_slices.argtypes = [A, A, A, A, A, W, A, A, vp]
     lib.rexlib_hip_insert_weighted_slices.argtypes = [A, A, A, A, A, A, W, A, A, vp]
     lib.rexlib_hip_insert_ctf_slices.argtypes = [A, A, A, A, A, A, W, O, ctypes.c_double, A, A, vp]
+    lib.rexlib_hip_insert_ctf_slices_no_alloc.argtypes = [A, A, A, A, A, A, W, O, ctypes.c_double, A, A, vp]
     lib.rexlib_hip_insert_ctf_images.argtypes = [A, A, A, A, ctypes.c_int64, A, A, A, W, O, ctypes.c_double, vp]
     lib.rexlib_hip_rfft2.argtypes = [A, A, vp]
     lib.rexlib_hip_irfft2.argtypes = [A, A, vp]
