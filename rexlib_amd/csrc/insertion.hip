@@ -364,49 +364,73 @@ void dispatch_scatter(const insertion_params &p, hipStream_t stream)
 // v * r * (numerator weight) and r * (denominator weight), or zeros when the
 // element is not inserted (row_insertion_kernel.inl:180-279).
 //
-// The brick is a box of signed frequencies (x in [0, N/2], y and z in
-// [-N/2, N/2]). The wavefront tests 64 slices at a time for planes that pass
-// near the brick; for each such slice it bounds the rows that reach the
-// brick and clips each row to the samples that reach it -- bound_rows and
-// clip_row of src/backends/cpu/kernels/row_interval.cpp, restated per brick
-// as the reference restates them per tile (slice_insertion_functor.inl:391-440).
+// Bricks tile the first-tap cells of the volume (signed x in [0, N/2], y and
+// z in [-N/2, N/2], each range extended downwards by taps - 1 so that every
+// sample reaching the volume has its first tap in some brick). A sample copy
+// belongs to the brick holding its first tap, so each copy is added by one
+// brick; the brick's LDS holds its cells and the taps - 1 voxels above them.
 //
-// A voxel takes 12 bytes of LDS (a 16 x 8 x 6 brick with its padding is
-// 10 KB), which lets 16 wavefronts share a CU: the adds of a tap wait on LDS
-// latency, and more wavefronts hide more of it.
+// The wavefront tests 64 slices at a time for planes that pass near the
+// brick; for each such slice it bounds the rows that can hold its samples and
+// clips each row to them -- bound_rows and clip_row of
+// src/backends/cpu/kernels/row_interval.cpp, restated per brick as the
+// reference restates them per tile (slice_insertion_functor.inl:391-440).
+// The rows of all hit slices are then packed densely over the lanes (64
+// rows at a time, each with its base position, step and first sample in a
+// small LDS table), and their samples over the lanes (64 at a time), so a
+// lane carries a sample whatever slice it comes from.
 //
 // The samples are added by plain LDS read-modify-write, not LDS float
 // atomics (which retire at about a third of a lane per clock per CU on
-// gfx950, against eight for a read-modify-write). A lane owns one sample per
-// instruction, and the samples are taken in four parity classes of
-// (k_x mod 2, k_y mod 2): two samples of a class differ by 2 a X + 2 b Y,
-// which is at least one voxel along some axis when the frame (X, Y) is close
-// enough to orthogonal with steps of at least about 0.87 voxel (every rotation
-// with N >= n), so no two lanes of an instruction touch one voxel. Slices
-// whose frames fail that test take LDS atomics instead. The wavefront is the
-// brick's only writer, so there is no barrier anywhere.
+// gfx950, against eight for a read-modify-write). Per round every lane
+// holding a sample claims its first-tap cell (a byte per cell: it writes its
+// lane, then reads it back); the lanes that read their own lane add their
+// taps one after another, the others keep their sample for the next round,
+// whose new samples go to the free lanes only. No two lanes of one tap
+// instruction then touch one voxel (a tap is a constant offset from the
+// cell). LDS runs a wavefront's accesses in program order and lds_order
+// keeps the compiler from moving them; the wavefront is the brick's only
+// writer, so there is no barrier anywhere.
+//
+// The volume phase exp(-2 pi i g C / N) depends on the voxel alone, so the
+// brick sums v * w and applies the phase once per voxel when it adds the
+// brick to the accumulators (alternating_sign.inl:10-20,
+// tabulated_phase.inl:12-37).
 //
 // Every plane passes through the origin, so bricks near it see every slice
 // and bricks far out see few (about 13 / r of them). A task is a brick and a
 // share of the slices: heavy bricks are split, and the heaviest tasks start
-// first. A brick that one task owns is added to the accumulators by plain
-// read-add-write; split bricks, and bricks holding the y or z frequency
-// +-N/2 of an even N (two signed frequencies, one stored element), add with
-// atomics.
+// first. Voxels that only one task can touch are added to the accumulators
+// by plain read-add-write; the others (the taps - 1 layers at each face that
+// the neighbouring bricks also reach, every voxel of a split brick, and bricks
+// holding the y or z frequency +-N/2 of an even N, two signed frequencies of
+// one stored element) add with atomics.
 // ---------------------------------------------------------------------------
 
 constexpr int brick_edge = 16; // along x; the extents along y and z are parameters
-// Slots of 64 samples whose loads a wavefront issues together.
-constexpr int max_slots = 4;
-// LDS pitches of a brick: odd, so that lanes stepping along any axis fall in
-// different banks.
-constexpr int pitch_y = brick_edge + 1;
 
-__host__ __device__ constexpr int brick_pitch_z(int extent_y)
+// Taps per axis an LDS brick is laid out for: the window's own (box 1,
+// triangle 2), at most 4 for Kaiser-Bessel windows on this schedule.
+template <int PROFILE>
+struct brick_reach
 {
-	return pitch_y * extent_y + 1;
-}
+	static constexpr int taps = PROFILE == REXLIB_HIP_BOX ? 1 : (PROFILE == REXLIB_HIP_TRIANGLE ? 2 : 4);
+};
 
+// LDS layout of a brick of first-tap cells 16 x EY x EZ with the taps - 1
+// voxels above them: odd pitches, so that lanes stepping along any axis fall
+// in different banks.
+template <int T, int EY, int EZ>
+struct brick_layout
+{
+	static constexpr int lx = brick_edge + T - 1;
+	static constexpr int ly = EY + T - 1;
+	static constexpr int lz = EZ + T - 1;
+	static constexpr int pitch_y = lx % 2 == 0 ? lx + 1 : lx;
+	static constexpr int pitch_z = (pitch_y * ly) % 2 == 0 ? pitch_y * ly + 1 : pitch_y * ly;
+	static constexpr int voxels = pitch_z * lz;
+	static constexpr int cells = brick_edge * EY * EZ;
+};
 
 // A voxel of a brick in LDS: 12 bytes.
 struct brick_voxel
@@ -418,8 +442,8 @@ struct brick_voxel
 
 struct brick_box
 {
-	int lo[3];  // signed (x, y, z) of the first voxel
-	int ext[3]; // voxels along (x, y, z)
+	int lo[3];  // signed (x, y, z) of the first cell
+	int ext[3]; // cells along (x, y, z)
 	bool alias_free;
 };
 
@@ -632,259 +656,6 @@ __global__ __launch_bounds__(256) void columns_prepare_kernel(insertion_params p
 	}
 }
 
-// Box and triangle windows on an even N: per sample, the first tap's brick
-// index, its volume phase (-1)^(x+y+z) (a wrapped index keeps its parity)
-// and the in-brick bits of the taps; each tap is then a constant offset and a
-// constant sign flip. The sums are those of the general form below.
-template <int PROFILE, bool ATOMIC, int EY, int EZ>
-__device__ __forceinline__ void add_sample_even(const insertion_params &p, const brick_box &box, brick_voxel *acc,
-	const float *c, float vr, float vi, float weight)
-{
-	constexpr int taps = window_taps<PROFILE>::taps;
-	constexpr int pitch_z = brick_pitch_z(EY);
-	float w[3][taps];
-	bool in[3][taps];
-	int g[3];
-	int local[3];
-#pragma unroll
-	for (int a = 0; a < 3; ++a)
-	{
-		if constexpr (PROFILE == REXLIB_HIP_TRIANGLE)
-		{
-			// floor and fraction, as window_taps<TRIANGLE>::place.
-			const float below = floorf(c[a]);
-			g[a] = static_cast<int>(below);
-			w[a][1] = c[a] - below;
-			w[a][0] = 1.0f - w[a][1];
-		}
-		else
-		{
-			g[a] = window_taps<PROFILE>::place(p, c[a], w[a]);
-		}
-		local[a] = g[a] - box.lo[a];
-#pragma unroll
-		for (int t = 0; t < taps; ++t)
-		{
-			in[a][t] = static_cast<unsigned>(local[a] + t) < static_cast<unsigned>(box.ext[a]);
-		}
-	}
-	const int base = local[0] + pitch_y * local[1] + pitch_z * local[2];
-	const bool odd = ((g[0] + g[1] + g[2]) & 1) != 0;
-	const float sr = odd ? -vr : vr;
-	const float si = odd ? -vi : vi;
-#pragma unroll
-	for (int tz = 0; tz < taps; ++tz)
-	{
-#pragma unroll
-		for (int ty = 0; ty < taps; ++ty)
-		{
-			const float wzy = w[2][tz] * w[1][ty];
-			const bool in_zy = in[2][tz] && in[1][ty];
-#pragma unroll
-			for (int tx = 0; tx < taps; ++tx)
-			{
-				const float ww = wzy * w[0][tx];
-				const bool flip = ((tx + ty + tz) & 1) != 0;
-				const float re = (flip ? -sr : sr) * ww;
-				const float im = (flip ? -si : si) * ww;
-				const int idx = base + tx + pitch_y * ty + pitch_z * tz;
-				if (in_zy && in[0][tx])
-				{
-					if (ATOMIC)
-					{
-						atomicAdd(&acc[idx].re, re);
-						atomicAdd(&acc[idx].im, im);
-						atomicAdd(&acc[idx].den, weight * ww);
-					}
-					else
-					{
-						brick_voxel v = acc[idx];
-						v.re += re;
-						v.im += im;
-						v.den += weight * ww;
-						acc[idx] = v;
-					}
-				}
-				lds_order();
-			}
-		}
-	}
-}
-
-// add_sample_even for a slot that holds rows of one k_y parity with every
-// k_x, consecutive k_x of a row in consecutive lanes. Two samples of such a
-// slot fall in one voxel cell only if they are neighbours in a row (see
-// insert_slice_sign), so a lane whose cell is that of the lane before it
-// hands its sample to that lane, which adds both in one pass over the taps:
-// the read-modify-write of a tap then never has two lanes on one voxel.
-// Every lane of the wavefront calls it (inactive ones with active false).
-template <int PROFILE, int EY, int EZ>
-__device__ __forceinline__ void add_sample_pair(const insertion_params &p, const brick_box &box, brick_voxel *acc,
-	const float *c, bool active, float vr, float vi, float weight)
-{
-	static_assert(PROFILE != REXLIB_HIP_KAISER_BESSEL, "box and triangle windows only");
-	constexpr int taps = window_taps<PROFILE>::taps;
-	constexpr int pitch_z = brick_pitch_z(EY);
-	const int lane = threadIdx.x & 63;
-	float w[3][taps];
-	bool in[3][taps];
-	int g[3];
-	int local[3];
-#pragma unroll
-	for (int a = 0; a < 3; ++a)
-	{
-		if constexpr (PROFILE == REXLIB_HIP_TRIANGLE)
-		{
-			const float below = floorf(c[a]);
-			g[a] = static_cast<int>(below);
-			w[a][1] = c[a] - below;
-			w[a][0] = 1.0f - w[a][1];
-		}
-		else
-		{
-			g[a] = window_taps<PROFILE>::place(p, c[a], w[a]);
-		}
-		local[a] = g[a] - box.lo[a];
-#pragma unroll
-		for (int t = 0; t < taps; ++t)
-		{
-			in[a][t] = static_cast<unsigned>(local[a] + t) < static_cast<unsigned>(box.ext[a]);
-		}
-	}
-	// The cell of an active sample, which lies within a few voxels of the
-	// brick, in 10 bits per axis (never 0); inactive lanes hold keys no other
-	// lane has. The lanes at the ends read 0 for their missing neighbour.
-	const int key = active ? (((local[0] + 512) & 1023) | (((local[1] + 512) & 1023) << 10) |
-		(((local[2] + 512) & 1023) << 20)) : ((1 << 30) | lane);
-	const bool merge = next_lane(key) == key;
-	const bool follower = previous_lane(key) == key;
-	// The next lane's sample (zero unless this lane takes it) and weights.
-	const float vr_next = next_lane(vr);
-	const float vi_next = next_lane(vi);
-	const float weight_next = next_lane(weight);
-	float w_next[3][taps];
-#pragma unroll
-	for (int a = 0; a < 3; ++a)
-	{
-		if constexpr (PROFILE == REXLIB_HIP_TRIANGLE)
-		{
-			w_next[a][1] = next_lane(w[a][1]);
-			w_next[a][0] = 1.0f - w_next[a][1];
-		}
-		else
-		{
-			w_next[a][0] = 1.0f;
-		}
-	}
-	const float nr = merge ? vr_next : 0.0f;
-	const float ni = merge ? vi_next : 0.0f;
-	const float nd = merge ? weight_next : 0.0f;
-	// A follower's sample is added by the lane before it.
-#pragma unroll
-	for (int t = 0; t < taps; ++t)
-	{
-		in[2][t] = in[2][t] && !follower;
-	}
-	const int base = local[0] + pitch_y * local[1] + pitch_z * local[2];
-	const bool odd = ((g[0] + g[1] + g[2]) & 1) != 0;
-	const float sr = odd ? -vr : vr;
-	const float si = odd ? -vi : vi;
-	const float tr = odd ? -nr : nr;
-	const float ti = odd ? -ni : ni;
-#pragma unroll
-	for (int tz = 0; tz < taps; ++tz)
-	{
-#pragma unroll
-		for (int ty = 0; ty < taps; ++ty)
-		{
-			const float wzy = w[2][tz] * w[1][ty];
-			const float wzy_next = w_next[2][tz] * w_next[1][ty];
-			const bool in_zy = in[2][tz] && in[1][ty];
-#pragma unroll
-			for (int tx = 0; tx < taps; ++tx)
-			{
-				const float ww = wzy * w[0][tx];
-				const float ww_next = wzy_next * w_next[0][tx];
-				const bool flip = ((tx + ty + tz) & 1) != 0;
-				const int idx = base + tx + pitch_y * ty + pitch_z * tz;
-				if (in_zy && in[0][tx])
-				{
-					// Two fused multiply-adds per component: the partner's term,
-					// then this lane's.
-					brick_voxel v = acc[idx];
-					v.re = fmaf(flip ? -sr : sr, ww, fmaf(flip ? -tr : tr, ww_next, v.re));
-					v.im = fmaf(flip ? -si : si, ww, fmaf(flip ? -ti : ti, ww_next, v.im));
-					v.den = fmaf(weight, ww, fmaf(nd, ww_next, v.den));
-					acc[idx] = v;
-				}
-				lds_order();
-			}
-		}
-	}
-}
-
-template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
-__device__ inline void add_sample(const insertion_params &p, const brick_box &box, brick_voxel *acc, const float *c,
-	float vr, float vi, float weight)
-{
-	if constexpr (!ODD && PROFILE != REXLIB_HIP_KAISER_BESSEL)
-	{
-		add_sample_even<PROFILE, ATOMIC, EY, EZ>(p, box, acc, c, vr, vi, weight);
-		return;
-	}
-	constexpr int pitch_z = brick_pitch_z(EY);
-	float wx[window_taps<PROFILE>::taps];
-	float wy[window_taps<PROFILE>::taps];
-	float wz[window_taps<PROFILE>::taps];
-	const int gx0 = window_taps<PROFILE>::place(p, c[0], wx);
-	const int gy0 = window_taps<PROFILE>::place(p, c[1], wy);
-	const int gz0 = window_taps<PROFILE>::place(p, c[2], wz);
-	const int taps = PROFILE == REXLIB_HIP_KAISER_BESSEL ? p.taps : window_taps<PROFILE>::taps;
-	const int big = p.big;
-	for (int tz = 0; tz < taps; ++tz)
-	{
-		const int lz = gz0 + tz - box.lo[2];
-		const bool in_z = static_cast<unsigned>(lz) < static_cast<unsigned>(box.ext[2]);
-		const int gz = gz0 + tz;
-		const int sz = gz < 0 ? gz + big : gz;
-		for (int ty = 0; ty < taps; ++ty)
-		{
-			const int ly = gy0 + ty - box.lo[1];
-			const bool in_zy = in_z && static_cast<unsigned>(ly) < static_cast<unsigned>(box.ext[1]);
-			const int gy = gy0 + ty;
-			const int sy = gy < 0 ? gy + big : gy;
-			const float wzy = wz[tz] * wy[ty];
-			for (int tx = 0; tx < taps; ++tx)
-			{
-				const int lx = gx0 + tx - box.lo[0];
-				const bool inside = in_zy && static_cast<unsigned>(lx) < static_cast<unsigned>(box.ext[0]);
-				const int gx = gx0 + tx;
-				const float w = wzy * wx[tx];
-				const float2 contribution = volume_phase<ODD>(p, make_float2(vr * w, vi * w), gx, sy, sz);
-				const int idx = lx + pitch_y * ly + pitch_z * lz;
-				if (inside)
-				{
-					if (ATOMIC)
-					{
-						atomicAdd(&acc[idx].re, contribution.x);
-						atomicAdd(&acc[idx].im, contribution.y);
-						atomicAdd(&acc[idx].den, weight * w);
-					}
-					else
-					{
-						brick_voxel v = acc[idx];
-						v.re += contribution.x;
-						v.im += contribution.y;
-						v.den += weight * w;
-						acc[idx] = v;
-					}
-				}
-				lds_order();
-			}
-		}
-	}
-}
-
 // bound_rows of one slice and sign: the rows [r0, r1) whose samples k_x in
 // [first, widest] can reach the box [lower, upper]; r0 >= r1 when none.
 // Evaluated per lane, one slice per lane.
@@ -943,336 +714,93 @@ __device__ inline void bound_rows(const insertion_params &p, const float *x_step
 	r1 = min(static_cast<int>(floorf(high)) + 1, last_row + 1);
 }
 
-// Position of sample k_x of row k_y along one axis, the sign applied to the
-// frame: the row base k_y * Y formed in double and rounded to float, plus
-// k_x * X in float, uncontracted (row_insertion_kernel.inl:157-166). Lanes
-// without a sample sit far outside every brick.
-__device__ inline float row_position(bool active, double y_step, float x_step, int kx, int ky)
+// Position of sample k_x along one axis from its row's base and step (the
+// sign applied): the row base k_y * Y is formed in double and rounded to
+// float once per row, then base + k_x * X in float, uncontracted
+// (row_insertion_kernel.inl:157-166).
+__device__ inline float row_position(float base, float step, int kx)
 {
 #pragma clang fp contract(off)
-	const float base = static_cast<float>(static_cast<double>(ky) * y_step);
-	const float at = base + static_cast<float>(kx) * x_step;
-	return active ? at : -1.0e6f;
+	const float product = static_cast<float>(kx) * step;
+	return base + product;
 }
 
-// One slice, one sign, into a brick, over the rows [r0, r1) that reach it;
-// the whole wavefront takes part.
-// `inv_x` holds 1 / x_step per axis (0 where the step is 0); `y_step_d` the
-// row step in double, which the positions use (y_step_f, its float, only
-// bounds the rows).
-template <int PROFILE, bool ODD, bool ATOMIC, int EY, int EZ>
-__device__ void insert_slice_sign(const insertion_params &p, const float *x_step, const float *y_step_f,
-	const double *y_step_d, const float *inv_x, int last_shell, const prepared_sample *slice, const brick_box &box,
-	const float *lower, const float *upper, int sign_index, int r0, int r1, brick_voxel *acc, int *row_table)
+// clip_row of one row k_y and sign: the samples k_x in [begin, end) (end <=
+// begin when none) within the shell radius whose positions can fall in
+// [lower, upper], a box widened by 0.01 voxel beyond every rounding here.
+// `inv_step` holds 1 / step per axis (0 where the step is 0); a product with
+// it stands for the division.
+__device__ inline void clip_row(int ky, int first, int widest, float radius, const float *step, const float *y_step,
+	const float *inv_step, const float *lower, const float *upper, int &begin, int &end)
 {
-	const int lane = threadIdx.x & 63;
-	const int n = p.n;
-	const int h = p.h;
-	const bool even = n % 2 == 0;
-	const int last_column = h - (even ? 2 : 1);
-	const float radius = static_cast<float>(last_shell) + 0.5f;
-	const int widest = min(last_column, last_shell);
-	const float sign = sign_index == 0 ? 1.0f : -1.0f;
-	const int first = sign_index;
-	float step[3];
-	float y_step[3];
-	double y_exact[3];
-	float inv_step[3];
+	begin = 0;
+	end = 0;
+	const float kyf = static_cast<float>(ky);
+	const float room = radius * radius - kyf * kyf;
+	if (room < 0.0f)
+	{
+		return;
+	}
+	const int last = min(widest, static_cast<int>(sqrtf(room)));
+	float lo_k = static_cast<float>(first);
+	float hi_k = static_cast<float>(last);
+	bool none = first > last;
+#pragma unroll
 	for (int a = 0; a < 3; ++a)
 	{
-		step[a] = sign * x_step[a];
-		y_step[a] = sign * y_step_f[a];
-		y_exact[a] = sign_index == 0 ? y_step_d[a] : -y_step_d[a];
-		inv_step[a] = sign * inv_x[a];
+		const float base = kyf * y_step[a];
+		if (step[a] == 0.0f)
+		{
+			none = none || !(lower[a] <= base && base <= upper[a]);
+		}
+		else
+		{
+			const float l = (lower[a] - base) * inv_step[a];
+			const float u = (upper[a] - base) * inv_step[a];
+			lo_k = fmaxf(lo_k, fminf(l, u));
+			hi_k = fminf(hi_k, fmaxf(l, u));
+		}
 	}
-	for (int rc = r0; rc < r1; rc += 64)
+	if (!none && lo_k <= hi_k)
 	{
-		// clip_row: the samples of row k_y that can reach the box.
-		const int ky = rc + lane;
-		int begin = 0;
-		int end = 0;
-		if (ky < r1)
-		{
-			const float kyf = static_cast<float>(ky);
-			const float room = radius * radius - kyf * kyf;
-			if (room >= 0.0f)
-			{
-				const int last = min(widest, static_cast<int>(sqrtf(room)));
-				float lo_k = static_cast<float>(first);
-				float hi_k = static_cast<float>(last);
-				bool none = first > last;
-#pragma unroll
-				for (int a = 0; a < 3; ++a)
-				{
-					const float base = kyf * y_step[a];
-					if (step[a] == 0.0f)
-					{
-						none = none || !(lower[a] <= base && base <= upper[a]);
-					}
-					else
-					{
-						// Reciprocal steps: the box is widened by 0.01 voxel,
-						// far beyond this rounding.
-						const float l = (lower[a] - base) * inv_step[a];
-						const float u = (upper[a] - base) * inv_step[a];
-						lo_k = fmaxf(lo_k, fminf(l, u));
-						hi_k = fminf(hi_k, fmaxf(l, u));
-					}
-				}
-				if (!none && lo_k <= hi_k)
-				{
-					begin = max(static_cast<int>(ceilf(lo_k)), first);
-					end = max(begin, min(static_cast<int>(floorf(hi_k)) + 1, last + 1));
-				}
-			}
-		}
-		// Box and triangle windows on an even N, lattice-free slices: the two
-		// classes of one k_y parity share a slot whenever both fit. Samples of
-		// one slot then differ by a X + 2 b Y; unless (a, b) = (+-1, 0) that
-		// is at least sqrt(4 * 0.76) > sqrt(3) voxels (the bound behind
-		// lattice_free), more than a voxel cell spans, so only neighbours in a
-		// row can share a cell, and add_sample_pair merges those.
-		if constexpr (!ODD && PROFILE != REXLIB_HIP_KAISER_BESSEL && !ATOMIC)
-		{
-			// Per row: every k_x of [begin, end), counted for its k_y parity.
-			const int row_count = max(end - begin, 0);
-			const int pair_count[2] = {(ky & 1) == 0 ? row_count : 0, (ky & 1) != 0 ? row_count : 0};
-			int pair_start[2];
-			int pair_total[2];
-			scan_two(pair_count[0], pair_count[1], pair_start[0], pair_start[1], pair_total[0], pair_total[1]);
-			if (pair_total[0] <= 64 && pair_total[1] <= 64)
-			{
-				prepared_sample q[2];
-				int sample_x[2];
-				int sample_y[2];
-				bool valid[2];
-#pragma unroll
-				for (int c = 0; c < 2; ++c)
-				{
-					q[c] = prepared_sample{0.0f, 0.0f, 0.0f};
-					sample_x[c] = 0;
-					sample_y[c] = 0;
-					valid[c] = false;
-					if (pair_total[c] == 0)
-					{
-						continue;
-					}
-					row_table[lane] = 0;
-					lds_order();
-					if (pair_count[c] > 0)
-					{
-						atomicMax(&row_table[pair_start[c] & 63], lane);
-					}
-					lds_order();
-					const int r = wave_prefix_max(row_table[lane]) & 63;
-					lds_order();
-					const int row_begin = __shfl(begin, r, 64);
-					const int row_start = __shfl(pair_start[c], r, 64);
-					const int sky = rc + r;
-					const int kx = row_begin + (lane - row_start);
-					const int row = sky < 0 ? sky + n : sky;
-					sample_x[c] = kx;
-					sample_y[c] = sky;
-					valid[c] = lane < pair_total[c];
-					q[c] = slice[valid[c] ? min(max(row * h + kx, 0), n * h - 1) : 0];
-				}
-#pragma unroll
-				for (int c = 0; c < 2; ++c)
-				{
-					if (pair_total[c] == 0)
-					{
-						continue;
-					}
-					const bool active = valid[c] && !(q[c].re == 0.0f && q[c].im == 0.0f && q[c].den == 0.0f);
-					float position[3];
-					for (int a = 0; a < 3; ++a)
-					{
-						position[a] = row_position(active, y_exact[a], step[a], sample_x[c], sample_y[c]);
-					}
-					add_sample_pair<PROFILE, EY, EZ>(p, box, acc, position, active, q[c].re,
-						sign_index == 0 ? q[c].im : -q[c].im, q[c].den);
-				}
-				continue;
-			}
-		}
-		// Four parity classes; a class holds rows of one parity of k_y and the
-		// k_x of one parity within them. Each class is packed densely over the
-		// lanes (a scan over the rows), in slots of 64 samples.
-		int class_begin[4];
-		int class_start[4];
-		int class_total[4];
-		int slot_end[4]; // running count of slots through each class
-		int count[4];
-		for (int cls = 0; cls < 4; ++cls)
-		{
-			const int py = cls >> 1;
-			const int px = cls & 1;
-			class_begin[cls] = begin + ((begin - px) & 1); // first k_x >= begin of parity px
-			count[cls] = 0;
-			if (((ky - py) & 1) == 0 && class_begin[cls] < end)
-			{
-				count[cls] = (end - class_begin[cls] + 1) / 2;
-			}
-		}
-		scan_two(count[0], count[1], class_start[0], class_start[1], class_total[0], class_total[1]);
-		scan_two(count[2], count[3], class_start[2], class_start[3], class_total[2], class_total[3]);
-		int slots = 0;
-		for (int cls = 0; cls < 4; ++cls)
-		{
-			slots += (class_total[cls] + 63) >> 6;
-			slot_end[cls] = slots;
-		}
-		// The common case, every class within one slot: class c is slot c,
-		// and the row of each sample comes from the wave's row table (each
-		// non-empty row records its lane at its first sample, then a prefix
-		// maximum over the lanes) rather than from a search. Every load index
-		// is checked against the slice before use.
-		if (class_total[0] <= 64 && class_total[1] <= 64 && class_total[2] <= 64 && class_total[3] <= 64)
-		{
-			prepared_sample q[4];
-			int sample_x[4];
-			int sample_y[4];
-			bool valid[4];
-#pragma unroll
-			for (int c = 0; c < 4; ++c)
-			{
-				q[c] = prepared_sample{0.0f, 0.0f, 0.0f};
-				sample_x[c] = 0;
-				sample_y[c] = 0;
-				valid[c] = false;
-				if (class_total[c] == 0)
-				{
-					continue;
-				}
-				// The first sample of a class always starts a row, so lane 0
-				// holds a row index of at least 0 and the prefix maximum
-				// gives every sample lane the row it falls in.
-				row_table[lane] = 0;
-				lds_order();
-				if (count[c] > 0)
-				{
-					atomicMax(&row_table[class_start[c] & 63], lane);
-				}
-				lds_order();
-				const int r = wave_prefix_max(row_table[lane]) & 63;
-				lds_order();
-				const int row_begin = __shfl(class_begin[c], r, 64);
-				const int row_start = __shfl(class_start[c], r, 64);
-				const int sky = rc + r;
-				const int kx = row_begin + 2 * (lane - row_start);
-				const int row = sky < 0 ? sky + n : sky;
-				sample_x[c] = kx;
-				sample_y[c] = sky;
-				valid[c] = lane < class_total[c];
-				// Clamped into the slice whatever the lane holds.
-				q[c] = slice[valid[c] ? min(max(row * h + kx, 0), n * h - 1) : 0];
-			}
-#pragma unroll
-			for (int c = 0; c < 4; ++c)
-			{
-				if (class_total[c] == 0)
-				{
-					continue;
-				}
-				// Lanes without a sample, and zero samples (not inserted), stay
-				// in step with the other lanes and add nothing.
-				const bool active = valid[c] && !(q[c].re == 0.0f && q[c].im == 0.0f && q[c].den == 0.0f);
-				float position[3];
-				for (int a = 0; a < 3; ++a)
-				{
-					position[a] = row_position(active, y_exact[a], step[a], sample_x[c], sample_y[c]);
-				}
-				add_sample<PROFILE, ODD, ATOMIC, EY, EZ>(p, box, acc, position, q[c].re,
-					sign_index == 0 ? q[c].im : -q[c].im, q[c].den);
-			}
-			continue;
-		}
-		// Gather, then scatter: the samples of up to max_slots slots are
-		// located and loaded first, so that their loads overlap, and added
-		// afterwards one slot at a time (one class per instruction).
-		for (int group = 0; group < slots; group += max_slots)
-		{
-			prepared_sample q[max_slots];
-			int sample_x[max_slots];
-			int sample_y[max_slots];
-#pragma unroll
-			for (int k = 0; k < max_slots; ++k)
-			{
-				q[k] = prepared_sample{0.0f, 0.0f, 0.0f};
-				sample_x[k] = 0;
-				sample_y[k] = 0;
-				const int g = group + k;
-				if (g >= slots)
-				{
-					continue;
-				}
-				const int cls = g < slot_end[0] ? 0 : (g < slot_end[1] ? 1 : (g < slot_end[2] ? 2 : 3));
-				const int first_slot = cls == 0 ? 0 : slot_end[cls - 1];
-				const int start = class_start[cls];
-				const int i = ((g - first_slot) << 6) + lane;
-				// The row holding sample i: the last lane whose start is at or below i.
-				int lo_r = 0;
-				int hi_r = 63;
-				for (int step_r = 0; step_r < 6; ++step_r)
-				{
-					const int mid = (lo_r + hi_r + 1) >> 1;
-					const int at = __shfl(start, mid, 64);
-					if (at <= i)
-					{
-						lo_r = mid;
-					}
-					else
-					{
-						hi_r = mid - 1;
-					}
-				}
-				const int row_begin = __shfl(class_begin[cls], lo_r, 64);
-				const int row_start = __shfl(start, lo_r, 64);
-				const int sky = rc + lo_r;
-				const int kx = row_begin + 2 * (i - row_start);
-				sample_x[k] = kx;
-				sample_y[k] = sky;
-				if (i < class_total[cls])
-				{
-					const int row = sky < 0 ? sky + n : sky;
-					q[k] = slice[row * h + kx];
-				}
-			}
-#pragma unroll
-			for (int k = 0; k < max_slots; ++k)
-			{
-				if (group + k >= slots)
-				{
-					continue;
-				}
-				// Zero samples (not inserted) stay in step with the other lanes.
-				const bool active = !(q[k].re == 0.0f && q[k].im == 0.0f && q[k].den == 0.0f);
-				float c[3];
-				for (int a = 0; a < 3; ++a)
-				{
-					c[a] = row_position(active, y_exact[a], step[a], sample_x[k], sample_y[k]);
-				}
-				add_sample<PROFILE, ODD, ATOMIC, EY, EZ>(p, box, acc, c, q[k].re, sign_index == 0 ? q[k].im : -q[k].im,
-					q[k].den);
-			}
-		}
+		begin = max(static_cast<int>(ceilf(lo_k)), first);
+		end = max(begin, min(static_cast<int>(floorf(hi_k)) + 1, last + 1));
 	}
 }
 
+// A wavefront owns the brick of one task (a brick and a share of the
+// slices): 16 x EY x EZ first-tap cells and the voxels their taps reach, in
+// LDS.
 template <int PROFILE, bool ODD, int EY, int EZ>
 __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chunk chunk, const prepared_sample *prepared,
-	const int4 *tasks, int task_count, float reach)
+	const int4 *tasks, int task_count)
 {
-	constexpr int pitch_z = brick_pitch_z(EY);
-	constexpr int voxels_max = pitch_z * EZ;
-	__shared__ brick_voxel acc[voxels_max];
-	// Per sample slot: the row of each sample (see insert_slice_sign).
-	__shared__ int row_table[64];
+	constexpr int T = brick_reach<PROFILE>::taps;
+	using L = brick_layout<T, EY, EZ>;
+	__shared__ brick_voxel acc[L::voxels];
+	// Per first-tap cell: the lane that last claimed it.
+	__shared__ unsigned char claim[L::cells];
+	// The current 64 rows: base position and first sample number; step and
+	// first k_x; the prepared element of (row, k_x = 0), bit 31 set for the
+	// -k copy.
+	__shared__ float4 row_base[64];
+	__shared__ float4 row_step[64];
+	__shared__ uint32_t row_index[64];
+	// Per lane: the last row (or slice) starting at or before it.
+	__shared__ int marker[64];
 
 	const int lane = threadIdx.x;
 	const int big = p.big;
-	const int64_t elements = static_cast<int64_t>(p.n) * p.h;
+	const int n = p.n;
+	const int h = p.h;
+	const int64_t elements = static_cast<int64_t>(n) * h;
 	const int64_t kept = blockIdx.y;
+	const int taps = PROFILE == REXLIB_HIP_KAISER_BESSEL ? p.taps : T;
+	const int last_column = h - (n % 2 == 0 ? 2 : 1);
+	// A first tap f holds the positions [f + offset, f + 1 + offset).
+	const float cell_offset = PROFILE == REXLIB_HIP_BOX ? -0.5f :
+		(PROFILE == REXLIB_HIP_TRIANGLE ? 0.0f : p.kb_half_width - 1.0f);
+	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 
 	for (int ti = blockIdx.x; ti < task_count; ti += gridDim.x)
 	{
@@ -1280,14 +808,13 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 		const int4 share = tasks[2 * ti + 1];
 		const int64_t share_begin = chunk.count * share.x / share.y;
 		const int64_t share_end = chunk.count * (share.x + 1) / share.y;
-		const int voxels = box.ext[0] * box.ext[1] * box.ext[2];
-		for (int v = lane; v < voxels_max; v += 64)
+		for (int v = lane; v < L::voxels; v += 64)
 		{
 			acc[v] = brick_voxel{0.0f, 0.0f, 0.0f};
 		}
 		lds_order();
-		// The box of positions that can reach the brick, widened beyond the
-		// window so that rounding never drops a sample.
+		// The positions whose first tap lies in the brick, widened so that
+		// rounding never drops a sample (each sample is then tested exactly).
 		float lower[3];
 		float upper[3];
 		float centre[3];
@@ -1295,29 +822,185 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 		float half_diagonal2 = 0.0f;
 		for (int a = 0; a < 3; ++a)
 		{
-			lower[a] = static_cast<float>(box.lo[a]) - reach - 0.01f;
-			upper[a] = static_cast<float>(box.lo[a] + box.ext[a] - 1) + reach + 0.01f;
-			centre[a] = static_cast<float>(box.lo[a]) + 0.5f * static_cast<float>(box.ext[a] - 1);
-			half_extent[a] = 0.5f * static_cast<float>(box.ext[a] - 1) + reach + 0.02f;
+			lower[a] = static_cast<float>(box.lo[a]) + cell_offset - 0.01f;
+			upper[a] = static_cast<float>(box.lo[a] + box.ext[a]) + cell_offset + 0.01f;
+			centre[a] = 0.5f * (lower[a] + upper[a]);
+			half_extent[a] = 0.5f * (upper[a] - lower[a]) + 0.01f;
 			half_diagonal2 += half_extent[a] * half_extent[a];
 		}
 		const float half_diagonal = sqrtf(half_diagonal2);
 		const float centre_norm = sqrtf(centre[0] * centre[0] + centre[1] * centre[1] + centre[2] * centre[2]);
 
+		// The sample this lane holds, not yet added: value, cell (bits 0-23,
+		// bit 24 set while held) and per axis the fraction past the first tap
+		// (triangle) or the position (Kaiser-Bessel).
+		bool pending = false;
+		float held_re = 0.0f, held_im = 0.0f, held_den = 0.0f;
+		int held_cell = 0;
+		float held_w[3] = {0.0f, 0.0f, 0.0f};
+
+		// One round: free lanes take the next samples of the current rows
+		// (when `fill`), then every holding lane claims its cell and the
+		// winners add their taps.
+		int next = 0;
+		int total = 0;
+		int current = 0;
+		int row_start = 0;
+		int row_count = 0;
+		auto round = [&](bool fill)
+		{
+			if (fill)
+			{
+				const uint64_t free_mask = __ballot(!pending);
+				const int take = min(static_cast<int>(__popcll(free_mask)), total - next);
+				if (take > 0)
+				{
+					// The row of sample next + lane: rows starting in [next, next + 64)
+					// mark their slot, slot 0 starts with the row holding `next`.
+					marker[lane] = lane == 0 ? current : 0;
+					lds_order();
+					if (row_count > 0 && row_start >= next && row_start < next + 64)
+					{
+						marker[row_start - next] = lane;
+					}
+					lds_order();
+					const int r = wave_prefix_max(marker[lane]);
+					lds_order();
+					const float4 b = row_base[r];
+					const float4 st = row_step[r];
+					const uint32_t ix = row_index[r];
+					const int kx = __float_as_int(st.w) + (next + lane - __float_as_int(b.w));
+					const bool make = lane < take;
+					prepared_sample q{0.0f, 0.0f, 0.0f};
+					if (make)
+					{
+						q = prepared[(ix & 0x7fffffffu) + static_cast<uint32_t>(kx)];
+					}
+					const float c[3] = {row_position(b.x, st.x, kx), row_position(b.y, st.y, kx),
+						row_position(b.z, st.z, kx)};
+					int local[3];
+					float w[3];
+#pragma unroll
+					for (int a = 0; a < 3; ++a)
+					{
+						const int f = window_taps<PROFILE>::first(p, c[a]);
+						local[a] = f - box.lo[a];
+						w[a] = PROFILE == REXLIB_HIP_TRIANGLE ? c[a] - static_cast<float>(f) : c[a];
+					}
+					const bool own = static_cast<unsigned>(local[0]) < static_cast<unsigned>(box.ext[0]) &&
+						static_cast<unsigned>(local[1]) < static_cast<unsigned>(box.ext[1]) &&
+						static_cast<unsigned>(local[2]) < static_cast<unsigned>(box.ext[2]);
+					const bool value = !(q.re == 0.0f && q.im == 0.0f && q.den == 0.0f);
+					const int cell = make && own && value ?
+						(local[0] | (local[1] << 8) | (local[2] << 16) | (1 << 24)) : 0;
+					const float im = (ix >> 31) != 0 ? -q.im : q.im;
+					current = __builtin_amdgcn_readlane(r, take - 1);
+					// Free lane k (in lane order) takes sample next + k.
+					const int rank = static_cast<int>(__popcll(free_mask & below));
+					const int source = rank < take ? rank : 0;
+					const int got = __shfl(cell, source, 64);
+					const float got_re = __shfl(q.re, source, 64);
+					const float got_im = __shfl(im, source, 64);
+					const float got_den = __shfl(q.den, source, 64);
+					float got_w[3];
+#pragma unroll
+					for (int a = 0; a < 3; ++a)
+					{
+						got_w[a] = __shfl(w[a], source, 64);
+					}
+					if (!pending && rank < take && (got >> 24) != 0)
+					{
+						pending = true;
+						held_re = got_re;
+						held_im = got_im;
+						held_den = got_den;
+						held_cell = got;
+						held_w[0] = got_w[0];
+						held_w[1] = got_w[1];
+						held_w[2] = got_w[2];
+					}
+					next += take;
+				}
+			}
+			const int lx = held_cell & 0xff;
+			const int ly = (held_cell >> 8) & 0xff;
+			const int lz = (held_cell >> 16) & 0xff;
+			const int cell = lx + brick_edge * (ly + EY * lz);
+			// A plain LDS read after the store (lds_order keeps the compiler
+			// from reusing the stored value); a volatile pointer would turn it
+			// into a flat load, which is not ordered after the LDS store.
+			if (pending)
+			{
+				claim[cell] = static_cast<unsigned char>(lane);
+			}
+			lds_order();
+			const bool win = pending && claim[cell] == lane;
+			lds_order();
+			if (win)
+			{
+				const int base = lx + L::pitch_y * ly + L::pitch_z * lz;
+				float wx[T], wy[T], wz[T];
+				if constexpr (PROFILE == REXLIB_HIP_TRIANGLE)
+				{
+					wx[0] = 1.0f - held_w[0];
+					wx[1] = held_w[0];
+					wy[0] = 1.0f - held_w[1];
+					wy[1] = held_w[1];
+					wz[0] = 1.0f - held_w[2];
+					wz[1] = held_w[2];
+				}
+				else if constexpr (PROFILE == REXLIB_HIP_BOX)
+				{
+					wx[0] = 1.0f;
+					wy[0] = 1.0f;
+					wz[0] = 1.0f;
+				}
+				else
+				{
+					window_taps<PROFILE>::template place<T>(p, held_w[0], wx);
+					window_taps<PROFILE>::template place<T>(p, held_w[1], wy);
+					window_taps<PROFILE>::template place<T>(p, held_w[2], wz);
+				}
+#pragma unroll
+				for (int tz = 0; tz < T; ++tz)
+				{
+#pragma unroll
+					for (int ty = 0; ty < T; ++ty)
+					{
+						const float wzy = wz[tz] * wy[ty];
+#pragma unroll
+						for (int tx = 0; tx < T; ++tx)
+						{
+							if (tx < taps && ty < taps && tz < taps)
+							{
+								const float wt = wzy * wx[tx];
+								brick_voxel &a = acc[base + tx + L::pitch_y * ty + L::pitch_z * tz];
+								brick_voxel v = a;
+								v.re += held_re * wt;
+								v.im += held_im * wt;
+								v.den += held_den * wt;
+								a = v;
+							}
+							lds_order();
+						}
+					}
+				}
+			}
+			lds_order();
+			pending = pending && !win;
+		};
+
 		for (int64_t window = share_begin; window < share_end; window += 64)
 		{
-			// Slices whose plane passes within reach of the brick, one per lane.
+			// Slices whose plane passes near the brick, one per lane.
 			const int64_t j = window + lane;
 			bool near = false;
-			// The frame of this lane's slice, kept for the insertions below.
 			float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f, y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
 			int last_shell = -1;
-			int lattice_free = 0;
 			if (j < share_end)
 			{
 				const slice_entry &e = p.entries[kept * p.map.reduced_count + chunk.begin + j];
 				last_shell = e.last_shell;
-				lattice_free = e.lattice_free;
 				if (e.last_shell >= 0)
 				{
 					x0 = e.x_step[0];
@@ -1361,85 +1044,147 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 				ix2 = x2 == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(x2);
 				bound_rows(p, xs, ys, iy, last_shell, lower, upper, 0, r0_plus, r1_plus);
 				bound_rows(p, xs, ys, iy, last_shell, lower, upper, 1, r0_minus, r1_minus);
-				near = r0_plus < r1_plus || r0_minus < r1_minus;
 			}
-			unsigned long long mask = __ballot(near);
-			while (mask != 0)
+			const int plus_rows = near ? max(r1_plus - r0_plus, 0) : 0;
+			const int minus_rows = near ? max(r1_minus - r0_minus, 0) : 0;
+			const int rows = plus_rows + minus_rows;
+			const int rows_inclusive = wave_inclusive_scan(rows);
+			const int rows_first = rows_inclusive - rows;
+			const int total_rows = __builtin_amdgcn_readlane(rows_inclusive, 63);
+			for (int rb = 0; rb < total_rows; rb += 64)
 			{
-				const int bit = __ffsll(static_cast<long long>(mask)) - 1;
-				mask &= mask - 1;
-				const int64_t js = window + bit;
-				const prepared_sample *slice = prepared + (kept * chunk.count + js) * elements;
-				// The hit slice's frame, read from the lane that tested it.
-				const float x_step[3] = {lane_value(x0, bit), lane_value(x1, bit), lane_value(x2, bit)};
-				const float y_step[3] = {lane_value(y0, bit), lane_value(y1, bit), lane_value(y2, bit)};
-				const float inv_x[3] = {lane_value(ix0, bit), lane_value(ix1, bit), lane_value(ix2, bit)};
-				// The row step in double, from the slice's entry (uniform).
-				const slice_entry &hit = p.entries[kept * p.map.reduced_count + chunk.begin + js];
-				const double y_step_d[3] = {hit.y_step[0], hit.y_step[1], hit.y_step[2]};
-				const int shell = __builtin_amdgcn_readlane(last_shell, bit);
-				const bool exact = __builtin_amdgcn_readlane(lattice_free, bit) != 0;
-				for (int sign_index = 0; sign_index < 2; ++sign_index)
+				// The slice of row rb + lane: every slice with rows in
+				// [rb, rb + 64) marks the slot of its first one (slot 0 for
+				// the slice holding row rb); a prefix maximum hands each row
+				// its slice's lane.
+				marker[lane] = 0;
+				lds_order();
+				if (rows > 0 && rows_first < rb + 64 && rows_first + rows > rb)
 				{
-					const int r0 = __builtin_amdgcn_readlane(sign_index == 0 ? r0_plus : r0_minus, bit);
-					const int r1 = __builtin_amdgcn_readlane(sign_index == 0 ? r1_plus : r1_minus, bit);
-					if (r0 >= r1)
+					marker[max(rows_first - rb, 0)] = lane;
+				}
+				lds_order();
+				const int s = wave_prefix_max(marker[lane]);
+				lds_order();
+				const int r = rb + lane;
+				// Every shuffle runs with all lanes active: a lane reads nothing
+				// from a source lane that is switched off.
+				const int idx = r - __shfl(rows_first, s, 64);
+				const int np = __shfl(plus_rows, s, 64);
+				const int first_plus = __shfl(r0_plus, s, 64);
+				const int first_minus = __shfl(r0_minus, s, 64);
+				const bool minus = idx >= np;
+				const int ky = minus ? first_minus + (idx - np) : first_plus + idx;
+				const float sign = minus ? -1.0f : 1.0f;
+				const float frame_x[3] = {__shfl(x0, s, 64), __shfl(x1, s, 64), __shfl(x2, s, 64)};
+				const float frame_y[3] = {__shfl(y0, s, 64), __shfl(y1, s, 64), __shfl(y2, s, 64)};
+				const float frame_ix[3] = {__shfl(ix0, s, 64), __shfl(ix1, s, 64), __shfl(ix2, s, 64)};
+				const int shell = __shfl(last_shell, s, 64);
+				const int64_t js = window + s;
+				float step[3];
+				float y_step[3];
+				float inv_step[3];
+				float base[3];
+				int begin = 0;
+				int end = 0;
+				if (r < total_rows)
+				{
+					const slice_entry &e = p.entries[kept * p.map.reduced_count + chunk.begin + js];
+					for (int a = 0; a < 3; ++a)
 					{
-						continue;
+						step[a] = sign * frame_x[a];
+						y_step[a] = sign * frame_y[a];
+						inv_step[a] = sign * frame_ix[a];
+						base[a] = static_cast<float>(static_cast<double>(ky) *
+							(minus ? -e.y_step[a] : e.y_step[a]));
 					}
-					if (exact)
+					clip_row(ky, minus ? 1 : 0, min(last_column, shell), static_cast<float>(shell) + 0.5f, step, y_step,
+						inv_step, lower, upper, begin, end);
+				}
+				else
+				{
+					for (int a = 0; a < 3; ++a)
 					{
-						insert_slice_sign<PROFILE, ODD, false, EY, EZ>(p, x_step, y_step, y_step_d, inv_x, shell, slice, box,
-							lower, upper, sign_index, r0, r1, acc, row_table);
-					}
-					else
-					{
-						insert_slice_sign<PROFILE, ODD, true, EY, EZ>(p, x_step, y_step, y_step_d, inv_x, shell, slice, box,
-							lower, upper, sign_index, r0, r1, acc, row_table);
+						step[a] = 0.0f;
+						base[a] = 0.0f;
 					}
 				}
+				row_count = end - begin;
+				const int count_inclusive = wave_inclusive_scan(row_count);
+				row_start = count_inclusive - row_count;
+				total = __builtin_amdgcn_readlane(count_inclusive, 63);
+				const int stored_row = ky < 0 ? ky + n : ky;
+				row_base[lane] = make_float4(base[0], base[1], base[2], __int_as_float(row_start));
+				row_step[lane] = make_float4(step[0], step[1], step[2], __int_as_float(begin));
+				row_index[lane] = r < total_rows ? (static_cast<uint32_t>((kept * chunk.count + js) * elements +
+					static_cast<int64_t>(stored_row) * h) | (minus ? 0x80000000u : 0u)) : 0u;
+				lds_order();
+				next = 0;
+				current = 0;
+				while (next < total)
+				{
+					round(true);
+				}
+				lds_order();
 			}
 		}
+		// Samples still held after the last rows.
+		while (__ballot(pending) != 0)
+		{
+			round(false);
+		}
 		lds_order();
+
 		// Add the brick to the accumulators of its kept batch element.
-		const bool exclusive = box.alias_free && share.y == 1;
+		// Voxels no other task touches (at least taps - 1 past the lower faces,
+		// inside the cells, a brick of one task without aliasing) take a plain
+		// read-add-write, the others atomics.
+		const bool single = share.y == 1 && box.alias_free;
+		const int he = big / 2;
 		const int64_t num_base = kept_offset(p.map, slot_numerator_out, kept);
 		const int64_t den_base = kept_offset(p.map, slot_denominator_out, kept);
-		for (int v = lane; v < voxels; v += 64)
+		for (int v = lane; v < L::voxels; v += 64)
 		{
-			const int lx = v % box.ext[0];
-			const int rest = v / box.ext[0];
-			const int ly = rest % box.ext[1];
-			const int lz = rest / box.ext[1];
-			const int idx = lx + pitch_y * ly + pitch_z * lz;
-			const brick_voxel sum = acc[idx];
-			const float re = sum.re;
-			const float im = sum.im;
-			const float d = sum.den;
-			if (re == 0.0f && im == 0.0f && d == 0.0f)
+			const int lz = v / L::pitch_z;
+			const int rest = v - lz * L::pitch_z;
+			const int ly = rest / L::pitch_y;
+			const int lx = rest - ly * L::pitch_y;
+			if (lx >= L::lx || ly >= L::ly)
 			{
 				continue;
 			}
-			const int gx = box.lo[0] + lx;
-			const int gy = box.lo[1] + ly;
-			const int gz = box.lo[2] + lz;
-			const int sy = gy < 0 ? gy + big : gy;
-			const int sz = gz < 0 ? gz + big : gz;
-			float2 *target = p.numerator + num_base + sz * p.num_s0 + sy * p.num_s1 + gx * p.num_s2;
-			float *dtarget = p.denominator + den_base + sz * p.den_s0 + sy * p.den_s1 + gx * p.den_s2;
+			const brick_voxel sum = acc[v];
+			if (sum.re == 0.0f && sum.im == 0.0f && sum.den == 0.0f)
+			{
+				continue;
+			}
+			const int x = box.lo[0] + lx;
+			const int y = box.lo[1] + ly;
+			const int z = box.lo[2] + lz;
+			if (x < 0 || x > he || y < -he || y > he || z < -he || z > he)
+			{
+				continue;
+			}
+			const int sy = y < 0 ? y + big : y;
+			const int sz = z < 0 ? z + big : z;
+			const float2 value = volume_phase<ODD>(p, make_float2(sum.re, sum.im), x, sy, sz);
+			float2 *target = p.numerator + num_base + sz * p.num_s0 + sy * p.num_s1 + x * p.num_s2;
+			float *dtarget = p.denominator + den_base + sz * p.den_s0 + sy * p.den_s1 + x * p.den_s2;
+			const bool exclusive = single && lx >= T - 1 && ly >= T - 1 && lz >= T - 1 && lx < box.ext[0] &&
+				ly < box.ext[1] && lz < box.ext[2];
 			if (exclusive)
 			{
 				float2 t = *target;
-				t.x += re;
-				t.y += im;
+				t.x += value.x;
+				t.y += value.y;
 				*target = t;
-				*dtarget += d;
+				*dtarget += sum.den;
 			}
 			else
 			{
-				atomicAdd(&target->x, re);
-				atomicAdd(&target->y, im);
-				atomicAdd(dtarget, d);
+				atomicAdd(&target->x, value.x);
+				atomicAdd(&target->y, value.y);
+				atomicAdd(dtarget, sum.den);
 			}
 		}
 		lds_order();
@@ -1481,60 +1226,73 @@ brick_tuning tuning_for(int extent)
 
 template <int PROFILE, int EY, int EZ>
 void launch_brick_shape(const insertion_params &p, slice_chunk chunk, const prepared_sample *prepared, const int4 *tasks,
-	int task_count, float reach, hipStream_t stream)
+	int task_count, hipStream_t stream)
 {
 	const dim3 grid(static_cast<unsigned>(task_count), static_cast<unsigned>(p.map.kept_count));
 	if (p.big % 2 == 1)
 	{
-		brick_kernel<PROFILE, true, EY, EZ><<<grid, 64, 0, stream>>>(p, chunk, prepared, tasks, task_count, reach);
+		brick_kernel<PROFILE, true, EY, EZ><<<grid, 64, 0, stream>>>(p, chunk, prepared, tasks, task_count);
 	}
 	else
 	{
-		brick_kernel<PROFILE, false, EY, EZ><<<grid, 64, 0, stream>>>(p, chunk, prepared, tasks, task_count, reach);
+		brick_kernel<PROFILE, false, EY, EZ><<<grid, 64, 0, stream>>>(p, chunk, prepared, tasks, task_count);
 	}
+}
+
+// Brick shape of a call: the tuned one, except that Kaiser-Bessel windows
+// (4 taps per axis) take 16 x 16 x 8 for 16 x 16 x 16, whose LDS would pass
+// 64 KB.
+brick_tuning launched_shape(int profile, const brick_tuning &tuning)
+{
+	brick_tuning t = tuning;
+	if (profile == REXLIB_HIP_KAISER_BESSEL && t.ey == 16 && t.ez == 16)
+	{
+		t.ez = 8;
+	}
+	return t;
 }
 
 template <int PROFILE>
 void launch_brick(const insertion_params &p, slice_chunk chunk, const prepared_sample *prepared, const int4 *tasks,
-	int task_count, float reach, const brick_tuning &tuning, hipStream_t stream)
+	int task_count, const brick_tuning &tuning, hipStream_t stream)
 {
 	const int ey = tuning.ey;
 	const int ez = tuning.ez;
 	if (ey == 8 && ez == 6)
 	{
-		launch_brick_shape<PROFILE, 8, 6>(p, chunk, prepared, tasks, task_count, reach, stream);
+		launch_brick_shape<PROFILE, 8, 6>(p, chunk, prepared, tasks, task_count, stream);
 	}
 	else if (ey == 8 && ez == 8)
 	{
-		launch_brick_shape<PROFILE, 8, 8>(p, chunk, prepared, tasks, task_count, reach, stream);
+		launch_brick_shape<PROFILE, 8, 8>(p, chunk, prepared, tasks, task_count, stream);
 	}
 	else if (ey == 8)
 	{
-		launch_brick_shape<PROFILE, 8, 16>(p, chunk, prepared, tasks, task_count, reach, stream);
+		launch_brick_shape<PROFILE, 8, 16>(p, chunk, prepared, tasks, task_count, stream);
 	}
 	else if (ez == 8)
 	{
-		launch_brick_shape<PROFILE, 16, 8>(p, chunk, prepared, tasks, task_count, reach, stream);
+		launch_brick_shape<PROFILE, 16, 8>(p, chunk, prepared, tasks, task_count, stream);
 	}
-	else
+	else if constexpr (PROFILE != REXLIB_HIP_KAISER_BESSEL)
 	{
-		launch_brick_shape<PROFILE, 16, 16>(p, chunk, prepared, tasks, task_count, reach, stream);
+		launch_brick_shape<PROFILE, 16, 16>(p, chunk, prepared, tasks, task_count, stream);
 	}
 }
 
 void dispatch_brick(const insertion_params &p, slice_chunk chunk, const prepared_sample *prepared, const int4 *tasks,
-	int task_count, float reach, const brick_tuning &tuning, hipStream_t stream)
+	int task_count, const brick_tuning &tuning, hipStream_t stream)
 {
 	switch (p.profile)
 	{
 	case REXLIB_HIP_BOX:
-		launch_brick<REXLIB_HIP_BOX>(p, chunk, prepared, tasks, task_count, reach, tuning, stream);
+		launch_brick<REXLIB_HIP_BOX>(p, chunk, prepared, tasks, task_count, tuning, stream);
 		break;
 	case REXLIB_HIP_TRIANGLE:
-		launch_brick<REXLIB_HIP_TRIANGLE>(p, chunk, prepared, tasks, task_count, reach, tuning, stream);
+		launch_brick<REXLIB_HIP_TRIANGLE>(p, chunk, prepared, tasks, task_count, tuning, stream);
 		break;
 	default:
-		launch_brick<REXLIB_HIP_KAISER_BESSEL>(p, chunk, prepared, tasks, task_count, reach, tuning, stream);
+		launch_brick<REXLIB_HIP_KAISER_BESSEL>(p, chunk, prepared, tasks, task_count, tuning, stream);
 		break;
 	}
 }
@@ -1626,11 +1384,19 @@ void dispatch_columns_prepare(const insertion_params &p, slice_chunk chunk, prep
 
 // The tasks of a brick grid as (packed box, (share, shares)) pairs,
 // heaviest first, for slice counts up to `bucket`.
-std::vector<int4> brick_task_items(int big, int64_t bucket, int target, int ey, int ez)
+std::vector<int4> brick_task_items(int big, int64_t bucket, int target, int ey, int ez, int taps)
 {
 	const int half = big / 2;
-	// Signed ranges: x in [0, N/2]; y, z in [-(N/2), N/2] (even N: N + 1
-	// values whose two ends are one stored element; odd N: N values).
+	// First-tap cells: x in [-(taps - 1), N/2]; y, z in [-(N/2) - (taps - 1),
+	// N/2] (even N: the ends -N/2 and N/2 are one stored element; odd N: N
+	// values from -(N/2)).
+	const int reach = taps - 1;
+	// Whether the voxels [first, first + count + taps - 1) hold -N/2 or N/2.
+	const auto aliased = [&](int first, int count)
+	{
+		const int end = first + count + reach;
+		return (first <= -half && -half < end) || (first <= half && half < end);
+	};
 	const auto starts = [](int low, int high, int edge)
 	{
 		std::vector<std::pair<int, int>> out;
@@ -1647,15 +1413,13 @@ std::vector<int4> brick_task_items(int big, int64_t bucket, int target, int ey, 
 		double load;
 	};
 	std::vector<item> items;
-	for (const auto &z : starts(-half, half, ez))
+	for (const auto &z : starts(-half - reach, half, ez))
 	{
-		for (const auto &y : starts(-half, half, ey))
+		for (const auto &y : starts(-half - reach, half, ey))
 		{
-			for (const auto &x : starts(0, half, brick_edge))
+			for (const auto &x : starts(-reach, half, brick_edge))
 			{
-				const bool alias = big % 2 == 0 &&
-					(z.first == -half || z.first + z.second - 1 == half || y.first == -half ||
-						y.first + y.second - 1 == half);
+				const bool alias = big % 2 == 0 && (aliased(z.first, z.second) || aliased(y.first, y.second));
 				const int w = x.second | (y.second << 8) | (z.second << 16) | ((alias ? 0 : 1) << 24);
 				const double cx = x.first + 0.5 * (x.second - 1);
 				const double cy = y.first + 0.5 * (y.second - 1);
@@ -1693,10 +1457,10 @@ struct task_list
 };
 
 std::mutex task_mutex;
-std::map<std::tuple<int, int, int64_t, int, int, int>, task_list> task_cache;
+std::map<std::tuple<int, int, int64_t, int, int, int, int>, task_list> task_cache;
 
 // The cached list for a call (made on first use); the caller holds task_mutex.
-task_list &brick_task_list(int big, int64_t slices, const brick_tuning &tuning)
+task_list &brick_task_list(int big, int64_t slices, const brick_tuning &tuning, int taps)
 {
 	int device = 0;
 	REXLIB_HIP_CHECK(hipGetDevice(&device));
@@ -1705,31 +1469,31 @@ task_list &brick_task_list(int big, int64_t slices, const brick_tuning &tuning)
 	{
 		bucket *= 2;
 	}
-	const auto key = std::make_tuple(device, big, bucket, tuning.share, tuning.ey, tuning.ez);
+	const auto key = std::make_tuple(device, big, bucket, tuning.share, tuning.ey, tuning.ez, taps);
 	auto found = task_cache.find(key);
 	if (found == task_cache.end())
 	{
 		found = task_cache.emplace(key, task_list{}).first;
-		found->second.host = brick_task_items(big, bucket, tuning.share, tuning.ey, tuning.ez);
+		found->second.host = brick_task_items(big, bucket, tuning.share, tuning.ey, tuning.ez, taps);
 	}
 	return found->second;
 }
 
 // Workspace bytes the task list of a call takes (with its alignment).
-std::size_t brick_task_bytes(int big, int64_t slices, const brick_tuning &tuning)
+std::size_t brick_task_bytes(int big, int64_t slices, const brick_tuning &tuning, int taps)
 {
 	std::lock_guard<std::mutex> lock(task_mutex);
-	return align_up(sizeof(int4) * brick_task_list(big, slices, tuning).host.size(), 256) + 256;
+	return align_up(sizeof(int4) * brick_task_list(big, slices, tuning, taps).host.size(), 256) + 256;
 }
 
 // The device copy of a call's tasks. With a fixed workspace buffer the list
 // is copied into it from the host copy the cache keeps (nothing allocated);
 // otherwise the device copy is made once and cached.
-const int4 *brick_tasks(int big, int64_t slices, const brick_tuning &tuning, workspace_source &workspace,
+const int4 *brick_tasks(int big, int64_t slices, const brick_tuning &tuning, int taps, workspace_source &workspace,
 	hipStream_t stream, int &count)
 {
 	std::lock_guard<std::mutex> lock(task_mutex);
-	task_list &list = brick_task_list(big, slices, tuning);
+	task_list &list = brick_task_list(big, slices, tuning, taps);
 	count = static_cast<int>(list.host.size() / 2);
 	const std::size_t bytes = sizeof(int4) * list.host.size();
 	if (workspace.arena != nullptr)
@@ -1839,7 +1603,9 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 	const int64_t per_slice = static_cast<int64_t>(p.n) * p.h * static_cast<int64_t>(sizeof(prepared_sample)) *
 		p.map.kept_count;
 	const int64_t limit = int64_t(4) << 30;
-	const brick_tuning tuning = tuning_for(p.big);
+	const brick_tuning tuning = launched_shape(p.profile, tuning_for(p.big));
+	// Taps per axis of the brick layout: 1, 2, or 4 for every Kaiser-Bessel window.
+	const int layout_taps = p.profile == REXLIB_HIP_BOX ? 1 : (p.profile == REXLIB_HIP_TRIANGLE ? 2 : 4);
 	int64_t chunk_count = 0;
 	if (schedule != schedule_scatter && p.taps <= 4 && p.map.kept_count <= 65535)
 	{
@@ -1856,7 +1622,7 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 			// Tasks made for more slices stay valid for fewer (their shares
 			// are fractions of a chunk).
 			chunk_count = std::min<int64_t>(chunk_count,
-				room(brick_task_bytes(p.big, chunk_count, tuning)) / per_slice);
+				room(brick_task_bytes(p.big, chunk_count, tuning, layout_taps)) / per_slice);
 		}
 	}
 	const bool brick = chunk_count >= 1;
@@ -1874,11 +1640,9 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 	if (brick)
 	{
 		int count = 0;
-		const int4 *tasks = brick_tasks(p.big, chunk_count, tuning, workspace, stream, count);
+		const int4 *tasks = brick_tasks(p.big, chunk_count, tuning, layout_taps, workspace, stream, count);
 		prepared_sample *prepared = static_cast<prepared_sample *>(workspace.take(area_prepared,
 			static_cast<std::size_t>(per_slice * chunk_count), stream));
-		const float reach = p.profile == REXLIB_HIP_BOX ? 0.5f : (p.profile == REXLIB_HIP_TRIANGLE ? 1.0f :
-			p.kb_half_width);
 		for (int64_t begin = 0; begin < p.map.reduced_count; begin += chunk_count)
 		{
 			const slice_chunk chunk{begin, std::min(chunk_count, p.map.reduced_count - begin)};
@@ -1894,7 +1658,7 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 			REXLIB_HIP_CHECK(hipGetLastError());
 			timing_end(prepare_token, stream);
 			const int brick_token = timing_begin(REXLIB_HIP_KERNEL_BRICK, stream);
-			dispatch_brick(p, chunk, prepared, tasks, count, reach, tuning, stream);
+			dispatch_brick(p, chunk, prepared, tasks, count, tuning, stream);
 			REXLIB_HIP_CHECK(hipGetLastError());
 			timing_end(brick_token, stream);
 		}
