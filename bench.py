@@ -57,8 +57,8 @@ def traffic_per_launch(args, tuned):
     if record.get("n") != args.n or record.get("N") != args.big or record.get("launch") != args.launch:
         return None
     brick = record.get("brick", {})
-    if tuned is None or ((brick.get("extent_y"), brick.get("extent_z")), brick.get("share")) != (tuple(tuned[0]),
-                                                                                              tuned[1]):
+    if tuned is None or ((brick.get("extent_x"), brick.get("extent_y"), brick.get("extent_z")),
+                         brick.get("share")) != (tuple(tuned[0]), tuned[1]):
         return None  # measured for another brick shape
     return record["bytes_per_launch"]
 
@@ -344,7 +344,7 @@ def main():
     # entry of this box size.
     tune.apply_all()
     tuned = tune.get_tuning(big)
-    log(f"brick tuning ((extent y, extent z), share): {tuned}")
+    log(f"brick tuning ((extent x, extent y, extent z), share): {tuned}")
 
     def step():
         num.zero_()
@@ -448,7 +448,8 @@ def main():
             "workload": (f"plain FP32: {args.images} images {n}^2 -> {big}^3, launches of {L}, fused rfft2 + "
                          "CTF trilinear insertion, then quotient, irfft3, gridding correction"),
             "images": args.images, "n": n, "N": big, "launch": L, "window": "linear", "variant": "ctf",
-            "brick": {"extent_y": tuned[0][0], "extent_z": tuned[0][1], "share": tuned[1]} if tuned else None,
+            "brick": {"extent_x": tuned[0][0], "extent_y": tuned[0][1], "extent_z": tuned[0][2],
+                      "share": tuned[1]} if tuned else None,
         },
         "roofline": roofline,
         "cpu_baseline": baseline,

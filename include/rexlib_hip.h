@@ -95,17 +95,20 @@ void rexlib_hip_version(int32_t *major, int32_t *minor, int32_t *patch);
 int32_t rexlib_hip_device_count(void);
 
 /* Tuning of the brick schedule of the insertions, per volume extent N
- * (extent 0: every extent without an entry of its own): the extents along y
- * and z of the brick a wavefront owns in LDS (16 along x) -- (8, 6), (8, 8),
- * (8, 16), (16, 8) or (16, 16) -- and the expected plane hits per task, in
- * [16, 2^20], that splits busy bricks into shares of their slices. Each
- * call reads its extent's entry once; without entries the default is
- * (8, 6) and 256. Returns REXLIB_HIP_INVALID_ARGUMENT for other values.
+ * (extent 0: every extent without an entry of its own): the first-tap cells
+ * along x, y and z of the brick a wavefront owns in LDS -- (16, 8, 6),
+ * (16, 8, 8), (16, 16, 8), (8, 8, 8), (8, 8, 6), (8, 8, 4) or (8, 6, 6) --
+ * and the expected plane hits per task, in [16, 2^20], that splits busy
+ * bricks into shares of their slices. Each call reads its extent's entry
+ * once; without entries the default is (16, 8, 6) and 256. Returns
+ * REXLIB_HIP_INVALID_ARGUMENT for other values.
  * No reference counterpart (the reference's CPU tuning is
  * slice_insertion_tuning, src/backends/cpu/builders/em/reconstruction/slice_insertion_tuning.cpp:10-13);
  * set by the autotuner (rexlib_amd/tune.py). */
-int rexlib_hip_set_insertion_tuning(int32_t extent, int32_t extent_y, int32_t extent_z, int32_t share);
-int rexlib_hip_get_insertion_tuning(int32_t extent, int32_t *extent_y, int32_t *extent_z, int32_t *share);
+int rexlib_hip_set_insertion_tuning(int32_t extent, int32_t extent_x, int32_t extent_y, int32_t extent_z,
+	int32_t share);
+int rexlib_hip_get_insertion_tuning(int32_t extent, int32_t *extent_x, int32_t *extent_y, int32_t *extent_z,
+	int32_t *share);
 /* Drops every entry (the defaults apply again). */
 void rexlib_hip_clear_insertion_tuning(void);
 

@@ -109,9 +109,9 @@ def library() -> ctypes.CDLL:
     lib.rexlib_hip_version.restype = None
     lib.rexlib_hip_device_count.restype = ctypes.c_int32
     lib.rexlib_hip_device_count.argtypes = []
-    lib.rexlib_hip_set_insertion_tuning.argtypes = [ctypes.c_int32] * 4
+    lib.rexlib_hip_set_insertion_tuning.argtypes = [ctypes.c_int32] * 5
     lib.rexlib_hip_set_insertion_tuning.restype = ctypes.c_int
-    lib.rexlib_hip_get_insertion_tuning.argtypes = [ctypes.c_int32] + [P(ctypes.c_int32)] * 3
+    lib.rexlib_hip_get_insertion_tuning.argtypes = [ctypes.c_int32] + [P(ctypes.c_int32)] * 4
     lib.rexlib_hip_get_insertion_tuning.restype = ctypes.c_int
     lib.rexlib_hip_clear_insertion_tuning.argtypes = []
     lib.rexlib_hip_clear_insertion_tuning.restype = None

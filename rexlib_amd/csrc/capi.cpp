@@ -490,21 +490,24 @@ void rexlib_hip_version(int32_t *major, int32_t *minor, int32_t *patch)
 	}
 }
 
-int rexlib_hip_set_insertion_tuning(int32_t extent, int32_t extent_y, int32_t extent_z, int32_t share)
+int rexlib_hip_set_insertion_tuning(int32_t extent, int32_t extent_x, int32_t extent_y, int32_t extent_z,
+	int32_t share)
 {
-	return guarded([&] { rexlib_hip::set_insertion_tuning(extent, extent_y, extent_z, share); });
+	return guarded([&] { rexlib_hip::set_insertion_tuning(extent, extent_x, extent_y, extent_z, share); });
 }
 
-int rexlib_hip_get_insertion_tuning(int32_t extent, int32_t *extent_y, int32_t *extent_z, int32_t *share)
+int rexlib_hip_get_insertion_tuning(int32_t extent, int32_t *extent_x, int32_t *extent_y, int32_t *extent_z,
+	int32_t *share)
 {
 	return guarded([&]
 		{
-			if (!extent_y || !extent_z || !share)
+			if (!extent_x || !extent_y || !extent_z || !share)
 			{
 				throw rexlib_hip::error(REXLIB_HIP_INVALID_ARGUMENT, "get_insertion_tuning: null output");
 			}
-			int ey = 0, ez = 0, sh = 0;
-			rexlib_hip::get_insertion_tuning(extent, &ey, &ez, &sh);
+			int ex = 0, ey = 0, ez = 0, sh = 0;
+			rexlib_hip::get_insertion_tuning(extent, &ex, &ey, &ez, &sh);
+			*extent_x = ex;
 			*extent_y = ey;
 			*extent_z = ez;
 			*share = sh;

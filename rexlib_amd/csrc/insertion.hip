@@ -407,7 +407,6 @@ void dispatch_scatter(const insertion_params &p, hipStream_t stream)
 // one stored element) add with atomics.
 // ---------------------------------------------------------------------------
 
-constexpr int brick_edge = 16; // along x; the extents along y and z are parameters
 
 // Taps per axis an LDS brick is laid out for: the window's own (box 1,
 // triangle 2), at most 4 for Kaiser-Bessel windows on this schedule.
@@ -417,22 +416,23 @@ struct brick_reach
 	static constexpr int taps = PROFILE == REXLIB_HIP_BOX ? 1 : (PROFILE == REXLIB_HIP_TRIANGLE ? 2 : 4);
 };
 
-// LDS layout of a brick of first-tap cells 16 x EY x EZ with the taps - 1
+// LDS layout of a brick of first-tap cells EX x EY x EZ with the taps - 1
 // voxels above them: odd pitches, so that lanes stepping along any axis fall
 // in different banks.
-template <int T, int EY, int EZ>
+template <int T, int EX, int EY, int EZ>
 struct brick_layout
 {
-	static constexpr int lx = brick_edge + T - 1;
+	static constexpr int lx = EX + T - 1;
 	static constexpr int ly = EY + T - 1;
 	static constexpr int lz = EZ + T - 1;
 	static constexpr int pitch_y = lx % 2 == 0 ? lx + 1 : lx;
 	static constexpr int pitch_z = (pitch_y * ly) % 2 == 0 ? pitch_y * ly + 1 : pitch_y * ly;
 	static constexpr int voxels = pitch_z * lz;
-	static constexpr int cells = brick_edge * EY * EZ;
+	static constexpr int cells = EX * EY * EZ;
 };
 
-// A voxel of a brick in LDS: 12 bytes.
+// A voxel of a brick in LDS: 12 bytes. (16-byte voxels, one 128-bit access
+// per tap, measured slower: the larger brick costs wavefronts per CU.)
 struct brick_voxel
 {
 	float re;
@@ -769,14 +769,14 @@ __device__ inline void clip_row(int ky, int first, int widest, float radius, con
 }
 
 // A wavefront owns the brick of one task (a brick and a share of the
-// slices): 16 x EY x EZ first-tap cells and the voxels their taps reach, in
+// slices): EX x EY x EZ first-tap cells and the voxels their taps reach, in
 // LDS.
-template <int PROFILE, bool ODD, int EY, int EZ>
+template <int PROFILE, bool ODD, int EX, int EY, int EZ>
 __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chunk chunk, const prepared_sample *prepared,
 	const int4 *tasks, int task_count)
 {
 	constexpr int T = brick_reach<PROFILE>::taps;
-	using L = brick_layout<T, EY, EZ>;
+	using L = brick_layout<T, EX, EY, EZ>;
 	__shared__ brick_voxel acc[L::voxels];
 	// Per first-tap cell: the lane that last claimed it.
 	__shared__ unsigned char claim[L::cells];
@@ -925,7 +925,7 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 			const int lx = held_cell & 0xff;
 			const int ly = (held_cell >> 8) & 0xff;
 			const int lz = (held_cell >> 16) & 0xff;
-			const int cell = lx + brick_edge * (ly + EY * lz);
+			const int cell = lx + EX * (ly + EY * lz);
 			// A plain LDS read after the store (lds_order keeps the compiler
 			// from reusing the stored value); a volatile pointer would turn it
 			// into a flat load, which is not ordered after the LDS store.
@@ -1191,26 +1191,30 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 	}
 }
 
-// Brick shapes: 16 x 8 x 6 (the default: 10 KB of LDS, 16 wavefronts per
-// CU), 16 x 8 x 8, 16 x 8 x 16, 16 x 16 x 8 and 16 x 16 x 16 (x is 16), and
-// the expected plane hits per task that splits busy bricks. One entry per
-// volume extent N (0: every extent without an entry of its own); a call
-// reads its extent's entry once, so concurrent callers each see a whole entry.
+// Brick shapes (first-tap cells along x, y, z): 16 x 8 x 6 (the default),
+// 16 x 8 x 8, 16 x 16 x 8, 8 x 8 x 8, 8 x 8 x 6, 8 x 6 x 6 and 8 x 8 x 4, and
+// the expected plane hits per task that splits busy bricks. Smaller bricks
+// take less LDS (more wavefronts per CU) but hold more halo voxels per cell.
+// One entry per volume extent N (0: every extent without an entry of its
+// own); a call reads its extent's entry once, so concurrent callers each see
+// a whole entry.
 struct brick_tuning
 {
+	int ex;
 	int ey;
 	int ez;
 	int share;
 };
 
-constexpr brick_tuning default_brick_tuning{8, 6, 256};
+constexpr brick_tuning default_brick_tuning{16, 8, 6, 256};
 
 std::mutex tuning_mutex;
 std::map<int, brick_tuning> tuning_table;
 
-bool valid_brick_shape(int ey, int ez)
+bool valid_brick_shape(int ex, int ey, int ez)
 {
-	return (ey == 8 && (ez == 6 || ez == 8 || ez == 16)) || (ey == 16 && (ez == 8 || ez == 16));
+	return (ex == 16 && ey == 8 && (ez == 6 || ez == 8)) || (ex == 16 && ey == 16 && ez == 8) ||
+		(ex == 8 && ey == 8 && (ez == 8 || ez == 6 || ez == 4)) || (ex == 8 && ey == 6 && ez == 6);
 }
 
 brick_tuning tuning_for(int extent)
@@ -1224,59 +1228,52 @@ brick_tuning tuning_for(int extent)
 	return found != tuning_table.end() ? found->second : default_brick_tuning;
 }
 
-template <int PROFILE, int EY, int EZ>
+template <int PROFILE, int EX, int EY, int EZ>
 void launch_brick_shape(const insertion_params &p, slice_chunk chunk, const prepared_sample *prepared, const int4 *tasks,
 	int task_count, hipStream_t stream)
 {
 	const dim3 grid(static_cast<unsigned>(task_count), static_cast<unsigned>(p.map.kept_count));
 	if (p.big % 2 == 1)
 	{
-		brick_kernel<PROFILE, true, EY, EZ><<<grid, 64, 0, stream>>>(p, chunk, prepared, tasks, task_count);
+		brick_kernel<PROFILE, true, EX, EY, EZ><<<grid, 64, 0, stream>>>(p, chunk, prepared, tasks, task_count);
 	}
 	else
 	{
-		brick_kernel<PROFILE, false, EY, EZ><<<grid, 64, 0, stream>>>(p, chunk, prepared, tasks, task_count);
+		brick_kernel<PROFILE, false, EX, EY, EZ><<<grid, 64, 0, stream>>>(p, chunk, prepared, tasks, task_count);
 	}
-}
-
-// Brick shape of a call: the tuned one, except that Kaiser-Bessel windows
-// (4 taps per axis) take 16 x 16 x 8 for 16 x 16 x 16, whose LDS would pass
-// 64 KB.
-brick_tuning launched_shape(int profile, const brick_tuning &tuning)
-{
-	brick_tuning t = tuning;
-	if (profile == REXLIB_HIP_KAISER_BESSEL && t.ey == 16 && t.ez == 16)
-	{
-		t.ez = 8;
-	}
-	return t;
 }
 
 template <int PROFILE>
 void launch_brick(const insertion_params &p, slice_chunk chunk, const prepared_sample *prepared, const int4 *tasks,
-	int task_count, const brick_tuning &tuning, hipStream_t stream)
+	int task_count, const brick_tuning &t, hipStream_t stream)
 {
-	const int ey = tuning.ey;
-	const int ez = tuning.ez;
-	if (ey == 8 && ez == 6)
+	if (t.ex == 16 && t.ey == 8 && t.ez == 8)
 	{
-		launch_brick_shape<PROFILE, 8, 6>(p, chunk, prepared, tasks, task_count, stream);
+		launch_brick_shape<PROFILE, 16, 8, 8>(p, chunk, prepared, tasks, task_count, stream);
 	}
-	else if (ey == 8 && ez == 8)
+	else if (t.ex == 16 && t.ey == 16)
 	{
-		launch_brick_shape<PROFILE, 8, 8>(p, chunk, prepared, tasks, task_count, stream);
+		launch_brick_shape<PROFILE, 16, 16, 8>(p, chunk, prepared, tasks, task_count, stream);
 	}
-	else if (ey == 8)
+	else if (t.ex == 8 && t.ey == 8 && t.ez == 8)
 	{
-		launch_brick_shape<PROFILE, 8, 16>(p, chunk, prepared, tasks, task_count, stream);
+		launch_brick_shape<PROFILE, 8, 8, 8>(p, chunk, prepared, tasks, task_count, stream);
 	}
-	else if (ez == 8)
+	else if (t.ex == 8 && t.ey == 8 && t.ez == 6)
 	{
-		launch_brick_shape<PROFILE, 16, 8>(p, chunk, prepared, tasks, task_count, stream);
+		launch_brick_shape<PROFILE, 8, 8, 6>(p, chunk, prepared, tasks, task_count, stream);
 	}
-	else if constexpr (PROFILE != REXLIB_HIP_KAISER_BESSEL)
+	else if (t.ex == 8 && t.ey == 8)
 	{
-		launch_brick_shape<PROFILE, 16, 16>(p, chunk, prepared, tasks, task_count, stream);
+		launch_brick_shape<PROFILE, 8, 8, 4>(p, chunk, prepared, tasks, task_count, stream);
+	}
+	else if (t.ex == 8)
+	{
+		launch_brick_shape<PROFILE, 8, 6, 6>(p, chunk, prepared, tasks, task_count, stream);
+	}
+	else
+	{
+		launch_brick_shape<PROFILE, 16, 8, 6>(p, chunk, prepared, tasks, task_count, stream);
 	}
 }
 
@@ -1384,7 +1381,7 @@ void dispatch_columns_prepare(const insertion_params &p, slice_chunk chunk, prep
 
 // The tasks of a brick grid as (packed box, (share, shares)) pairs,
 // heaviest first, for slice counts up to `bucket`.
-std::vector<int4> brick_task_items(int big, int64_t bucket, int target, int ey, int ez, int taps)
+std::vector<int4> brick_task_items(int big, int64_t bucket, int target, int ex, int ey, int ez, int taps)
 {
 	const int half = big / 2;
 	// First-tap cells: x in [-(taps - 1), N/2]; y, z in [-(N/2) - (taps - 1),
@@ -1417,7 +1414,7 @@ std::vector<int4> brick_task_items(int big, int64_t bucket, int target, int ey, 
 	{
 		for (const auto &y : starts(-half - reach, half, ey))
 		{
-			for (const auto &x : starts(-reach, half, brick_edge))
+			for (const auto &x : starts(-reach, half, ex))
 			{
 				const bool alias = big % 2 == 0 && (aliased(z.first, z.second) || aliased(y.first, y.second));
 				const int w = x.second | (y.second << 8) | (z.second << 16) | ((alias ? 0 : 1) << 24);
@@ -1457,7 +1454,7 @@ struct task_list
 };
 
 std::mutex task_mutex;
-std::map<std::tuple<int, int, int64_t, int, int, int, int>, task_list> task_cache;
+std::map<std::tuple<int, int, int64_t, int, int, int, int, int>, task_list> task_cache;
 
 // The cached list for a call (made on first use); the caller holds task_mutex.
 task_list &brick_task_list(int big, int64_t slices, const brick_tuning &tuning, int taps)
@@ -1469,12 +1466,12 @@ task_list &brick_task_list(int big, int64_t slices, const brick_tuning &tuning, 
 	{
 		bucket *= 2;
 	}
-	const auto key = std::make_tuple(device, big, bucket, tuning.share, tuning.ey, tuning.ez, taps);
+	const auto key = std::make_tuple(device, big, bucket, tuning.share, tuning.ex, tuning.ey, tuning.ez, taps);
 	auto found = task_cache.find(key);
 	if (found == task_cache.end())
 	{
 		found = task_cache.emplace(key, task_list{}).first;
-		found->second.host = brick_task_items(big, bucket, tuning.share, tuning.ey, tuning.ez, taps);
+		found->second.host = brick_task_items(big, bucket, tuning.share, tuning.ex, tuning.ey, tuning.ez, taps);
 	}
 	return found->second;
 }
@@ -1512,28 +1509,29 @@ const int4 *brick_tasks(int big, int64_t slices, const brick_tuning &tuning, int
 
 } // anonymous namespace
 
-void set_insertion_tuning(int extent, int extent_y, int extent_z, int share)
+void set_insertion_tuning(int extent, int extent_x, int extent_y, int extent_z, int share)
 {
 	if (extent < 0)
 	{
 		fail(REXLIB_HIP_INVALID_ARGUMENT, "insertion tuning: the extent must be 0 (every extent) or positive");
 	}
-	if (!valid_brick_shape(extent_y, extent_z))
+	if (!valid_brick_shape(extent_x, extent_y, extent_z))
 	{
-		fail(REXLIB_HIP_INVALID_ARGUMENT, "insertion tuning: brick extents (y, z) are (8, 6), (8, 8), (8, 16), "
-			"(16, 8) or (16, 16)");
+		fail(REXLIB_HIP_INVALID_ARGUMENT, "insertion tuning: brick extents (x, y, z) are (16, 8, 6), (16, 8, 8), "
+			"(16, 16, 8), (8, 8, 8), (8, 8, 6), (8, 8, 4) or (8, 6, 6)");
 	}
 	if (share < 16 || share > (1 << 20))
 	{
 		fail(REXLIB_HIP_INVALID_ARGUMENT, "insertion tuning: plane hits per task lie in [16, 2^20]");
 	}
 	std::lock_guard<std::mutex> lock(tuning_mutex);
-	tuning_table[extent] = brick_tuning{extent_y, extent_z, share};
+	tuning_table[extent] = brick_tuning{extent_x, extent_y, extent_z, share};
 }
 
-void get_insertion_tuning(int extent, int *extent_y, int *extent_z, int *share)
+void get_insertion_tuning(int extent, int *extent_x, int *extent_y, int *extent_z, int *share)
 {
 	const brick_tuning t = tuning_for(extent);
+	*extent_x = t.ex;
 	*extent_y = t.ey;
 	*extent_z = t.ez;
 	*share = t.share;
@@ -1603,7 +1601,7 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 	const int64_t per_slice = static_cast<int64_t>(p.n) * p.h * static_cast<int64_t>(sizeof(prepared_sample)) *
 		p.map.kept_count;
 	const int64_t limit = int64_t(4) << 30;
-	const brick_tuning tuning = launched_shape(p.profile, tuning_for(p.big));
+	const brick_tuning tuning = tuning_for(p.big);
 	// Taps per axis of the brick layout: 1, 2, or 4 for every Kaiser-Bessel window.
 	const int layout_taps = p.profile == REXLIB_HIP_BOX ? 1 : (p.profile == REXLIB_HIP_TRIANGLE ? 2 : 4);
 	int64_t chunk_count = 0;

@@ -2,7 +2,7 @@
 
 For a workload (slice extent n, volume extent N, slices per launch L) it times
 the insertion on the device for every brick shape the library holds (the
-brick's extents along y and z; 16 along x) and several shares (expected plane
+brick's first-tap cells along x, y and z) and several shares (expected plane
 hits per task, which split busy bricks), checks every configuration's sums
 against the first one's, and records the fastest per volume extent N in a
 JSON table. ``apply_all()`` hands the
@@ -25,10 +25,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
-# (y, z) extents of the bricks the library is built for.
-SHAPES = ((8, 6), (8, 8), (8, 16), (16, 8), (16, 16))
-SHARES = (64, 96, 128, 192, 256, 512)
-DEFAULT = {"extent_y": 8, "extent_z": 6, "share": 256}
+# (x, y, z) extents of the bricks the library is built for.
+SHAPES = ((16, 8, 6), (16, 8, 8), (16, 16, 8), (8, 8, 8), (8, 8, 6), (8, 8, 4), (8, 6, 6))
+SHARES = (64, 128, 256, 512, 1024)
+DEFAULT = {"extent_x": 16, "extent_y": 8, "extent_z": 6, "share": 256}
 
 
 def load_table(path: str = TABLE) -> dict:
@@ -41,15 +41,16 @@ def load_table(path: str = TABLE) -> dict:
 def set_tuning(extent: int, shape, share: int) -> None:
     """The library's entry for volumes of extent ``extent`` (0: every other extent)."""
     from . import _native as N
-    N.check(N.library().rexlib_hip_set_insertion_tuning(int(extent), int(shape[0]), int(shape[1]), int(share)))
+    N.check(N.library().rexlib_hip_set_insertion_tuning(int(extent), int(shape[0]), int(shape[1]), int(shape[2]),
+                                                         int(share)))
 
 
 def get_tuning(extent: int):
-    """((extent_y, extent_z), share) that calls with volume extent ``extent`` use."""
+    """((extent_x, extent_y, extent_z), share) that calls with volume extent ``extent`` use."""
     from . import _native as N
-    v = [ctypes.c_int32() for _ in range(3)]
+    v = [ctypes.c_int32() for _ in range(4)]
     N.check(N.library().rexlib_hip_get_insertion_tuning(int(extent), *[ctypes.byref(x) for x in v]))
-    return (v[0].value, v[1].value), v[2].value
+    return (v[0].value, v[1].value, v[2].value), v[3].value
 
 
 def clear_tuning() -> None:
@@ -61,7 +62,7 @@ def apply_all(path: str = TABLE) -> dict:
     """Hand every recorded entry (keyed by N) to the library; returns them."""
     table = load_table(path)
     for key, entry in table.items():
-        set_tuning(int(key), (entry["extent_y"], entry["extent_z"]), entry["share"])
+        set_tuning(int(key), (entry["extent_x"], entry["extent_y"], entry["extent_z"]), entry["share"])
     return table
 
 
@@ -131,21 +132,22 @@ def tune(n: int, big: int, launch: int, path: str = TABLE, verbose: bool = True)
                 if not error <= 1e-5:
                     raise RuntimeError(f"brick shape {shape}, share {share}: sums differ from "
                                        f"the first configuration's by rel-L2 {error:.3e}")
-                results.append({"extent_y": shape[0], "extent_z": shape[1], "share": share, "ms": ms,
-                                "rel_l2": error})
+                results.append({"extent_x": shape[0], "extent_y": shape[1], "extent_z": shape[2], "share": share,
+                                "ms": ms, "rel_l2": error})
                 if verbose:
-                    print(f"n={n} N={big} L={launch} brick 16x{shape[0]}x{shape[1]} share={share}: {ms:.3f} ms "
-                          f"(rel-L2 to the first {error:.1e})", flush=True)
+                    print(f"n={n} N={big} L={launch} brick {shape[0]}x{shape[1]}x{shape[2]} share={share}: "
+                          f"{ms:.3f} ms (rel-L2 to the first {error:.1e})", flush=True)
     finally:
         set_tuning(big, *previous)
     best = min(results, key=lambda r: r["ms"])
     table = load_table(path)
-    table[str(big)] = {"extent_y": best["extent_y"], "extent_z": best["extent_z"], "share": best["share"], "n": n,
+    table[str(big)] = {"extent_x": best["extent_x"], "extent_y": best["extent_y"], "extent_z": best["extent_z"],
+                       "share": best["share"], "n": n,
                        "launch": launch, "ms": best["ms"], "images_per_s": launch / (best["ms"] * 1e-3),
                        "tuned": time.strftime("%Y-%m-%d")}
     with open(path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
-    set_tuning(big, (best["extent_y"], best["extent_z"]), best["share"])
+    set_tuning(big, (best["extent_x"], best["extent_y"], best["extent_z"]), best["share"])
     return {"best": best, "all": results}
 
 

@@ -121,7 +121,7 @@ def test_every_brick_shape_and_share(big, tuning_table):
     try:
         for shape in tune.SHAPES:
             for share in SHARES:
-                print(f"N={big} brick 16x{shape[0]}x{shape[1]} share={share}", flush=True)
+                print(f"N={big} brick {shape} share={share}", flush=True)
                 tune.set_tuning(big, shape, share)
                 num, den = zeros(big)
                 em.insert_ctf_slices(num, den, *t[2:], em.GriddingWindow.linear(), OPTICS, 1.5,
@@ -144,7 +144,7 @@ def test_every_brick_shape_with_one_and_four_taps(window, tuning_table):
     t = [to_device(a) for a in inputs]
     try:
         for shape in tune.SHAPES:
-            print(f"{window.profile} brick 16x{shape[0]}x{shape[1]}", flush=True)
+            print(f"{window.profile} brick {shape}", flush=True)
             tune.set_tuning(big, shape, 64)
             num, den = zeros(big)
             em.insert_ctf_slices(num, den, *t[2:], hip_window(window), OPTICS, 1.5, numerator_out=num,

@@ -208,18 +208,18 @@ def test_insertion_tuning_is_a_table_per_volume_extent():
     lib = N.library()
     tune.clear_tuning()
     try:
-        assert tune.get_tuning(256) == ((8, 6), 256)
-        tune.set_tuning(256, (16, 8), 128)
-        assert tune.get_tuning(256) == ((16, 8), 128)
-        assert tune.get_tuning(64) == ((8, 6), 256)
-        tune.set_tuning(0, (8, 8), 64)
-        assert tune.get_tuning(64) == ((8, 8), 64)
-        assert tune.get_tuning(256) == ((16, 8), 128)
-        assert lib.rexlib_hip_set_insertion_tuning(96, 8, 7, 128) == N.INVALID_ARGUMENT
-        assert lib.rexlib_hip_set_insertion_tuning(96, 16, 6, 128) == N.INVALID_ARGUMENT
-        assert lib.rexlib_hip_set_insertion_tuning(96, 8, 6, 8) == N.INVALID_ARGUMENT
-        assert lib.rexlib_hip_set_insertion_tuning(-1, 8, 6, 64) == N.INVALID_ARGUMENT
-        assert tune.get_tuning(96) == ((8, 8), 64)
+        assert tune.get_tuning(256) == ((16, 8, 6), 256)
+        tune.set_tuning(256, (16, 16, 8), 128)
+        assert tune.get_tuning(256) == ((16, 16, 8), 128)
+        assert tune.get_tuning(64) == ((16, 8, 6), 256)
+        tune.set_tuning(0, (8, 8, 8), 64)
+        assert tune.get_tuning(64) == ((8, 8, 8), 64)
+        assert tune.get_tuning(256) == ((16, 16, 8), 128)
+        assert lib.rexlib_hip_set_insertion_tuning(96, 16, 8, 7, 128) == N.INVALID_ARGUMENT
+        assert lib.rexlib_hip_set_insertion_tuning(96, 8, 16, 6, 128) == N.INVALID_ARGUMENT
+        assert lib.rexlib_hip_set_insertion_tuning(96, 16, 8, 6, 8) == N.INVALID_ARGUMENT
+        assert lib.rexlib_hip_set_insertion_tuning(-1, 16, 8, 6, 64) == N.INVALID_ARGUMENT
+        assert tune.get_tuning(96) == ((8, 8, 8), 64)
     finally:
         tune.clear_tuning()
-    assert tune.get_tuning(256) == ((8, 6), 256)
+    assert tune.get_tuning(256) == ((16, 8, 6), 256)
