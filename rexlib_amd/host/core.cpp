@@ -147,7 +147,7 @@ strided_layout strided_layout::make_contiguous(std::vector<std::size_t> extents)
 	return strided_layout(std::move(extents), std::move(strides), 0);
 }
 
-std::size_t strided_layout::get_element_count() const noexcept
+std::size_t strided_layout::compute_element_count() const noexcept
 {
 	std::size_t count = 1;
 	for (std::size_t e : m_extents)
@@ -157,33 +157,71 @@ std::size_t strided_layout::get_element_count() const noexcept
 	return count;
 }
 
-bool strided_layout::is_contiguous() const noexcept
+namespace
 {
-	std::ptrdiff_t step = 1;
-	for (std::size_t i = m_extents.size(); i-- > 0;)
-	{
-		if (m_extents[i] != 1 && m_strides[i] != step)
-		{
-			return false;
-		}
-		step *= static_cast<std::ptrdiff_t>(m_extents[i]);
-	}
-	return true;
-}
 
-std::size_t strided_layout::get_span() const noexcept
+// Elements spanned from the offset to the last addressed element, plus one.
+std::size_t span_elements(const strided_layout &layout)
 {
-	if (get_element_count() == 0)
+	if (layout.compute_element_count() == 0)
 	{
 		return 0;
 	}
+	std::vector<std::size_t> extents;
+	std::vector<std::ptrdiff_t> strides;
+	layout.get_extents(extents);
+	layout.get_strides(strides);
 	std::ptrdiff_t last = 0;
-	for (std::size_t i = 0; i < m_extents.size(); ++i)
+	for (std::size_t i = 0; i < extents.size(); ++i)
 	{
-		last += static_cast<std::ptrdiff_t>(m_extents[i] - 1) * std::max<std::ptrdiff_t>(m_strides[i], 0);
+		last += static_cast<std::ptrdiff_t>(extents[i] - 1) * std::max<std::ptrdiff_t>(strides[i], 0);
 	}
 	return static_cast<std::size_t>(last + 1);
 }
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Device properties
+// ---------------------------------------------------------------------------
+
+struct device_properties::implementation
+{
+	device_type type = device_type::unknown;
+	std::string name;
+	std::string location;
+	std::size_t total_memory = 0;
+	std::size_t alignment = 256;
+};
+
+device_properties::device_properties() : m_implementation(new implementation) {}
+device_properties::~device_properties() = default;
+device_properties::device_properties(const device_properties &other)
+	: m_implementation(new implementation(*other.m_implementation))
+{
+}
+device_properties &device_properties::operator=(const device_properties &other)
+{
+	*m_implementation = *other.m_implementation;
+	return *this;
+}
+void device_properties::set_type(device_type type) { m_implementation->type = type; }
+device_type device_properties::get_type() const noexcept { return m_implementation->type; }
+void device_properties::set_name(const std::string &name) { m_implementation->name = name; }
+const std::string &device_properties::get_name() const noexcept { return m_implementation->name; }
+void device_properties::set_physical_location(const std::string &location) { m_implementation->location = location; }
+const std::string &device_properties::get_physical_location() const noexcept { return m_implementation->location; }
+void device_properties::set_total_memory_bytes(std::size_t bytes) { m_implementation->total_memory = bytes; }
+std::size_t device_properties::get_total_memory_bytes() const noexcept { return m_implementation->total_memory; }
+void device_properties::set_optimal_data_alignment(std::size_t alignment)
+{
+	if (alignment == 0 || (alignment & (alignment - 1)) != 0)
+	{
+		throw std::invalid_argument("device_properties: the alignment must be a power of two");
+	}
+	m_implementation->alignment = alignment;
+}
+std::size_t device_properties::get_optimal_data_alignment() const noexcept { return m_implementation->alignment; }
 
 // ---------------------------------------------------------------------------
 // Hardware abstraction: out-of-line constructors and destructors
@@ -689,10 +727,11 @@ public:
 					" has no planned layout");
 			}
 			const array_descriptor &d = planned[i];
-			const std::size_t size = d.get_layout().get_span() * get_size(d.get_data_type());
+			const std::size_t size = span_elements(d.get_layout()) * get_size(d.get_data_type());
 			const auto &allocator = device.get_allocator(memory_resource_affinity::device);
 			const std::size_t alignment = std::min({allocator->get_max_alignment(),
-				device.get_device_session()->get_properties().optimal_alignment, bit_ceil(std::max<std::size_t>(size, 1))});
+				device.get_device_session()->get_properties().get_optimal_data_alignment(),
+				bit_ceil(std::max<std::size_t>(size, 1))});
 			command_token pending;
 			outputs[i] = array(allocator->allocate(std::max<std::size_t>(size, 1), alignment, &queue, pending), d);
 			dependencies.push_back(pending);
@@ -761,7 +800,7 @@ execution_context make_execution_context(service_catalog &catalog, const device_
 array make_host_array(std::vector<std::size_t> extents, numerical_type type)
 {
 	strided_layout layout = strided_layout::make_contiguous(std::move(extents));
-	const std::size_t size = std::max<std::size_t>(layout.get_element_count() * get_size(type), 1);
+	const std::size_t size = std::max<std::size_t>(layout.compute_element_count() * get_size(type), 1);
 	command_token pending;
 	auto allocator = get_host_memory_resource().create_allocator();
 	return array(allocator->allocate(size, 64, nullptr, pending), array_descriptor(std::move(layout), type));
@@ -772,7 +811,7 @@ array zeros(std::vector<std::size_t> extents, numerical_type type, const executi
 	check_context(context);
 	array out;
 	const array_descriptor planned(strided_layout::make_contiguous(std::move(extents)), type);
-	context.get_dispatcher()->dispatch(ops::fill_operation(0.0), {&out, 1}, {&planned, 1}, {},
+	context.get_dispatcher()->dispatch(ops::fill_operation(scalar_value(0.0)), {&out, 1}, {&planned, 1}, {},
 		context.get_device_context());
 	return out;
 }
@@ -809,7 +848,7 @@ array full(std::vector<std::size_t> extents, double value, numerical_type type, 
 	check_context(context);
 	array out;
 	const array_descriptor planned(strided_layout::make_contiguous(std::move(extents)), type);
-	context.get_dispatcher()->dispatch(ops::fill_operation(value), {&out, 1}, {&planned, 1}, {},
+	context.get_dispatcher()->dispatch(ops::fill_operation(scalar_value(value)), {&out, 1}, {&planned, 1}, {},
 		context.get_device_context());
 	return out;
 }
@@ -956,16 +995,17 @@ namespace
 {
 
 // Axes as absolute indices, every axis when empty; repeated axes refused.
-ops::axis_list absolute_axes(ops::axis_list axes, std::size_t rank, const char *what)
+ops::axis_list absolute_axes(signed_axes given, std::size_t rank, const char *what)
 {
-	if (axes.empty())
+	if (given.empty())
 	{
 		for (std::size_t a = 0; a < rank; ++a)
 		{
-			axes.push_back(static_cast<std::ptrdiff_t>(a));
+			given.push_back(static_cast<std::ptrdiff_t>(a));
 		}
 	}
-	for (auto &a : axes)
+	ops::axis_list axes;
+	for (auto a : given)
 	{
 		if (a < 0)
 		{
@@ -975,6 +1015,7 @@ ops::axis_list absolute_axes(ops::axis_list axes, std::size_t rank, const char *
 		{
 			throw std::invalid_argument(std::string(what) + ": an axis lies outside the operand");
 		}
+		axes.push_back(static_cast<std::size_t>(a));
 	}
 	for (std::size_t i = 0; i < axes.size(); ++i)
 	{
@@ -1007,36 +1048,36 @@ numerical_type complex_result(numerical_type t, const char *what)
 
 } // namespace
 
-array fftn(const_array_ref x, ops::axis_list axes, const execution_context &context,
+array fftn(const_array_ref x, signed_axes given, const execution_context &context,
 	ops::fourier_normalization normalization, array *out)
 {
-	axes = absolute_axes(std::move(axes), x->get_extents().size(), "fftn");
+	const ops::axis_list axes = absolute_axes(std::move(given), x->get_extents().size(), "fftn");
 	return run_transform(ops::fft_operation(axes, normalization), x, x->get_extents(),
 		complex_result(x->get_data_type(), "fftn"), context, out);
 }
 
-array ifftn(const_array_ref x, ops::axis_list axes, const execution_context &context,
+array ifftn(const_array_ref x, signed_axes given, const execution_context &context,
 	ops::fourier_normalization normalization, array *out)
 {
-	axes = absolute_axes(std::move(axes), x->get_extents().size(), "ifftn");
+	const ops::axis_list axes = absolute_axes(std::move(given), x->get_extents().size(), "ifftn");
 	return run_transform(ops::ifft_operation(axes, normalization), x, x->get_extents(),
 		complex_result(x->get_data_type(), "ifftn"), context, out);
 }
 
-array rfftn(const_array_ref x, ops::axis_list axes, const execution_context &context,
+array rfftn(const_array_ref x, signed_axes given, const execution_context &context,
 	ops::fourier_normalization normalization, array *out)
 {
-	axes = absolute_axes(std::move(axes), x->get_extents().size(), "rfftn");
+	const ops::axis_list axes = absolute_axes(std::move(given), x->get_extents().size(), "rfftn");
 	std::vector<std::size_t> e = x->get_extents();
 	e[static_cast<std::size_t>(axes.back())] = e[static_cast<std::size_t>(axes.back())] / 2 + 1;
 	return run_transform(ops::rfft_operation(axes, normalization), x, e, complex_of(x->get_data_type()), context,
 		out);
 }
 
-array irfftn(const_array_ref x, ops::axis_list axes, std::size_t extent, const execution_context &context,
+array irfftn(const_array_ref x, signed_axes given, std::size_t extent, const execution_context &context,
 	ops::fourier_normalization normalization, array *out)
 {
-	axes = absolute_axes(std::move(axes), x->get_extents().size(), "irfftn");
+	const ops::axis_list axes = absolute_axes(std::move(given), x->get_extents().size(), "irfftn");
 	std::vector<std::size_t> e = x->get_extents();
 	const std::size_t last = static_cast<std::size_t>(axes.back());
 	if (e[last] != extent / 2 + 1)
@@ -1053,8 +1094,8 @@ array rfft2(const_array_ref x, ops::fourier_normalization normalization, const e
 	require_rank(x, 2, "rfft2");
 	std::vector<std::size_t> e = x->get_extents();
 	e.back() = e.back() / 2 + 1;
-	return run_transform(ops::rfft_operation({-2, -1}, normalization), x, e, complex_of(x->get_data_type()), context,
-		out);
+	return run_transform(ops::rfft_operation(absolute_axes({-2, -1}, e.size(), "rfft2"), normalization), x, e,
+		complex_of(x->get_data_type()), context, out);
 }
 
 array rfft2(const_array_ref x, const execution_context &context, array *out)
@@ -1071,7 +1112,8 @@ array irfft2(const_array_ref x, std::size_t extent, const execution_context &con
 	}
 	std::vector<std::size_t> e = x->get_extents();
 	e.back() = extent;
-	return run_transform(ops::irfft_operation({-2, -1}, extent), x, e, real_of(x->get_data_type()), context, out);
+	const ops::axis_list axes = absolute_axes({-2, -1}, e.size(), "irfft2");
+	return run_transform(ops::irfft_operation(axes, extent), x, e, real_of(x->get_data_type()), context, out);
 }
 
 array irfft3(const_array_ref x, std::size_t extent, ops::fourier_normalization normalization,
@@ -1084,8 +1126,9 @@ array irfft3(const_array_ref x, std::size_t extent, ops::fourier_normalization n
 	}
 	std::vector<std::size_t> e = x->get_extents();
 	e.back() = extent;
-	return run_transform(ops::irfft_operation({-3, -2, -1}, extent, normalization), x, e, real_of(x->get_data_type()),
-		context, out);
+	const ops::axis_list axes = absolute_axes({-3, -2, -1}, e.size(), "irfft3");
+	return run_transform(ops::irfft_operation(axes, extent, normalization), x, e, real_of(x->get_data_type()), context,
+		out);
 }
 
 array irfft3(const_array_ref x, std::size_t extent, const execution_context &context, array *out)

@@ -22,8 +22,43 @@
 // synchronises the stream before the error propagates. A builder returns
 // unsupported, never throws, for a queue, type or memory it cannot take, so
 // that another backend can take the operation.
-#include "rexlib/em.hpp"
-#include "rexlib/fourier.hpp"
+
+// rexlib's own headers, by the paths rexlib installs (built here against the
+// restatements under rexlib_amd/host/rexlib; tests/test_plugin_boundary.py
+// checks this file against rexlib's headers themselves).
+#include <rexlib/core/dispatch/operand_descriptor.hpp>
+#include <rexlib/core/dispatch/program_builder.hpp>
+#include <rexlib/core/dispatch/program_service.hpp>
+#include <rexlib/core/exceptions/unsupported_operation_error.hpp>
+#include <rexlib/core/hardware/buffer.hpp>
+#include <rexlib/core/hardware/command.hpp>
+#include <rexlib/core/hardware/command_queue.hpp>
+#include <rexlib/core/hardware/command_timeline.hpp>
+#include <rexlib/core/hardware/command_token.hpp>
+#include <rexlib/core/hardware/device.hpp>
+#include <rexlib/core/hardware/device_backend.hpp>
+#include <rexlib/core/hardware/device_properties.hpp>
+#include <rexlib/core/hardware/device_service.hpp>
+#include <rexlib/core/hardware/memory_allocator.hpp>
+#include <rexlib/core/hardware/memory_resource.hpp>
+#include <rexlib/core/hardware/program.hpp>
+#include <rexlib/core/layout/strided_layout.hpp>
+#include <rexlib/core/operation/operation_cast.hpp>
+#include <rexlib/core/plugin.hpp>
+#include <rexlib/core/plugin_definition.hpp>
+#include <rexlib/core/service_catalog.hpp>
+#include <rexlib/em/ops/reconstruction/insert_ctf_slices_operation.hpp>
+#include <rexlib/em/ops/reconstruction/insert_slices_operation.hpp>
+#include <rexlib/em/ops/reconstruction/insert_weighted_slices_operation.hpp>
+#include <rexlib/ops/arithmetic/divide_operation.hpp>
+#include <rexlib/ops/assignment/copy_operation.hpp>
+#include <rexlib/ops/assignment/fill_operation.hpp>
+#include <rexlib/ops/compare/greater_operation.hpp>
+#include <rexlib/ops/fourier/fft_operation.hpp>
+#include <rexlib/ops/fourier/ifft_operation.hpp>
+#include <rexlib/ops/fourier/irfft_operation.hpp>
+#include <rexlib/ops/fourier/rfft_operation.hpp>
+#include <rexlib/ops/selection/where_operation.hpp>
 
 #include "rexlib_hip.h"
 
@@ -367,9 +402,10 @@ public:
 		{
 			return false;
 		}
-		desc.name = std::string(p.name) + " (" + p.gcnArchName + ")";
-		desc.total_memory = p.totalGlobalMem;
-		desc.optimal_alignment = 256;
+		desc.set_type(device_type::gpu);
+		desc.set_name(std::string(p.name) + " (" + p.gcnArchName + ")");
+		desc.set_total_memory_bytes(p.totalGlobalMem);
+		desc.set_optimal_data_alignment(256);
 		return true;
 	}
 	std::shared_ptr<device> create_device(std::size_t id) const override
@@ -391,15 +427,19 @@ bool on_hip_queue(command_queue &queue)
 	return dynamic_cast<hip_command_queue *>(&queue) != nullptr;
 }
 
-bool device_accessible(const operand_descriptor &d)
+// An operand that the queue's device addresses: memory of this plugin's own
+// resources on that device (device-local, or the pinned host staging),
+// never another backend's or another device's.
+bool device_accessible(const operand_descriptor &d, const command_queue &queue)
 {
-	const memory_resource *r = d.get_memory_resource();
-	return r != nullptr && is_device_accessible(r->get_kind());
+	const auto *q = dynamic_cast<const hip_command_queue *>(&queue);
+	const auto *r = dynamic_cast<const hip_memory_resource *>(d.get_memory_resource());
+	return q != nullptr && r != nullptr && r->get_device() == q->get_device() && is_device_accessible(r->get_kind());
 }
 
-bool all_device_accessible(span<const operand_descriptor> ds)
+bool all_device_accessible(span<const operand_descriptor> ds, const command_queue &queue)
 {
-	return std::all_of(ds.begin(), ds.end(), device_accessible);
+	return std::all_of(ds.begin(), ds.end(), [&](const operand_descriptor &d) { return device_accessible(d, queue); });
 }
 
 bool types_are(span<const operand_descriptor> ds, std::initializer_list<numerical_type> types)
@@ -452,12 +492,42 @@ rexlib_hip_array abi_array(const operand_descriptor &d, const buffer &b)
 		layout.get_offset() * static_cast<std::ptrdiff_t>(get_size(d.get_data_type()));
 	a.dtype = abi_type(d.get_data_type());
 	a.rank = static_cast<int32_t>(layout.get_rank());
+	std::vector<std::size_t> extents;
+	std::vector<std::ptrdiff_t> strides;
+	layout.get_extents(extents);
+	layout.get_strides(strides);
 	for (std::size_t i = 0; i < layout.get_rank(); ++i)
 	{
-		a.extents[i] = static_cast<int64_t>(layout.get_extents()[i]);
-		a.strides[i] = static_cast<int64_t>(layout.get_strides()[i]);
+		a.extents[i] = static_cast<int64_t>(extents[i]);
+		a.strides[i] = static_cast<int64_t>(strides[i]);
 	}
 	return a;
+}
+
+// Extents of an operand, and whether its layout is row-major contiguous.
+std::vector<std::size_t> extents_of(const operand_descriptor &d)
+{
+	std::vector<std::size_t> e;
+	d.get_layout().get_extents(e);
+	return e;
+}
+
+bool contiguous(const strided_layout &layout)
+{
+	std::vector<std::size_t> extents;
+	std::vector<std::ptrdiff_t> strides;
+	layout.get_extents(extents);
+	layout.get_strides(strides);
+	std::ptrdiff_t step = 1;
+	for (std::size_t i = extents.size(); i-- > 0;)
+	{
+		if (extents[i] != 1 && strides[i] != step)
+		{
+			return false;
+		}
+		step *= static_cast<std::ptrdiff_t>(extents[i]);
+	}
+	return true;
 }
 
 rexlib_hip_window abi_window(const em::gridding_window &w)
@@ -508,20 +578,29 @@ protected:
 	std::vector<operand_descriptor> m_inputs;
 };
 
+// What an insertion program keeps of its operation (operations are not
+// copyable): the window, and for the CTF form the optics and pixel size.
+struct insertion_parameters
+{
+	rexlib_hip_window window;
+	rexlib_hip_ctf_optics optics;
+	double pixel_size;
+};
+
 template <typename Operation>
 class insertion_program final : public bound_program
 {
 public:
-	insertion_program(const Operation &op, span<const operand_descriptor> outputs,
+	insertion_program(const insertion_parameters &parameters, span<const operand_descriptor> outputs,
 		span<const operand_descriptor> inputs)
-		: bound_program(outputs, inputs), m_op(op)
+		: bound_program(outputs, inputs), m_parameters(parameters)
 	{
 	}
 	void enqueue(span<const std::shared_ptr<buffer>> outputs, span<const std::shared_ptr<const buffer>> inputs,
 		hipStream_t stream) const override;
 
 private:
-	Operation m_op;
+	insertion_parameters m_parameters;
 };
 
 template <>
@@ -530,8 +609,8 @@ void insertion_program<em::insert_slices_operation>::enqueue(span<const std::sha
 {
 	std::vector<rexlib_hip_array> out;
 	const auto in = arrays(outputs, inputs, out);
-	const rexlib_hip_window w = abi_window(m_op.get_window());
-	abi_check(rexlib_hip_insert_slices(&in[0], &in[1], &in[2], &in[3], &in[4], &w, &out[0], &out[1], stream));
+	abi_check(rexlib_hip_insert_slices(&in[0], &in[1], &in[2], &in[3], &in[4], &m_parameters.window, &out[0],
+		&out[1], stream));
 }
 
 template <>
@@ -540,9 +619,8 @@ void insertion_program<em::insert_weighted_slices_operation>::enqueue(span<const
 {
 	std::vector<rexlib_hip_array> out;
 	const auto in = arrays(outputs, inputs, out);
-	const rexlib_hip_window w = abi_window(m_op.get_window());
-	abi_check(rexlib_hip_insert_weighted_slices(&in[0], &in[1], &in[2], &in[3], &in[4], &in[5], &w, &out[0],
-		&out[1], stream));
+	abi_check(rexlib_hip_insert_weighted_slices(&in[0], &in[1], &in[2], &in[3], &in[4], &in[5], &m_parameters.window,
+		&out[0], &out[1], stream));
 }
 
 template <>
@@ -551,11 +629,23 @@ void insertion_program<em::insert_ctf_slices_operation>::enqueue(span<const std:
 {
 	std::vector<rexlib_hip_array> out;
 	const auto in = arrays(outputs, inputs, out);
-	const rexlib_hip_window w = abi_window(m_op.get_window());
-	const rexlib_hip_ctf_optics o{m_op.get_optics().get_voltage(), m_op.get_optics().get_spherical_aberration(),
-		m_op.get_optics().get_amplitude_contrast()};
-	abi_check(rexlib_hip_insert_ctf_slices(&in[0], &in[1], &in[2], &in[3], &in[4], &in[5], &w, &o,
-		m_op.get_pixel_size(), &out[0], &out[1], stream));
+	abi_check(rexlib_hip_insert_ctf_slices(&in[0], &in[1], &in[2], &in[3], &in[4], &in[5], &m_parameters.window,
+		&m_parameters.optics, m_parameters.pixel_size, &out[0], &out[1], stream));
+}
+
+template <typename Operation>
+insertion_parameters parameters_of(const Operation &op)
+{
+	return insertion_parameters{abi_window(op.get_window()), rexlib_hip_ctf_optics{}, 0.0};
+}
+
+template <>
+insertion_parameters parameters_of(const em::insert_ctf_slices_operation &op)
+{
+	const em::ctf_optics &o = op.get_optics();
+	return insertion_parameters{abi_window(op.get_window()),
+		rexlib_hip_ctf_optics{o.get_voltage(), o.get_spherical_aberration(), o.get_amplitude_contrast()},
+		op.get_pixel_size()};
 }
 
 // Input types of each insertion (accumulators, slices, then real operands).
@@ -596,7 +686,7 @@ public:
 	backend_priority get_suitability(const operation &, span<const operand_descriptor> outputs,
 		span<const operand_descriptor> inputs, command_queue &queue) const override
 	{
-		if (!on_hip_queue(queue) || !all_device_accessible(outputs) || !all_device_accessible(inputs) ||
+		if (!on_hip_queue(queue) || !all_device_accessible(outputs, queue) || !all_device_accessible(inputs, queue) ||
 			!types_are(outputs, {numerical_type::complex_float32, numerical_type::float32}) ||
 			!types_are(inputs, insertion_inputs<Operation>()))
 		{
@@ -607,7 +697,8 @@ public:
 	std::shared_ptr<program> build(const operation &op, span<const operand_descriptor> outputs,
 		span<const operand_descriptor> inputs, command_queue &, program_cache *) const override
 	{
-		return std::make_shared<insertion_program<Operation>>(operation_cast<Operation>(op), outputs, inputs);
+		return std::make_shared<insertion_program<Operation>>(parameters_of(operation_cast<Operation>(op)), outputs,
+			inputs);
 	}
 };
 
@@ -628,8 +719,9 @@ bool smooth_length(std::size_t n)
 	return n == 1;
 }
 
-// Normalised axes of an operation on `rank` axes: the last `count` or nothing.
-bool last_axes(const ops::axis_list &axes, std::size_t rank, std::size_t count)
+// Whether the (absolute) axes of an operation on `rank` axes are its last
+// `count`, in order.
+bool last_axes(span<const std::size_t> axes, std::size_t rank, std::size_t count)
 {
 	if (axes.size() != count || rank < count)
 	{
@@ -637,8 +729,7 @@ bool last_axes(const ops::axis_list &axes, std::size_t rank, std::size_t count)
 	}
 	for (std::size_t i = 0; i < count; ++i)
 	{
-		std::ptrdiff_t a = axes[i] < 0 ? axes[i] + static_cast<std::ptrdiff_t>(rank) : axes[i];
-		if (a != static_cast<std::ptrdiff_t>(rank - count + i))
+		if (axes[i] != rank - count + i)
 		{
 			return false;
 		}
@@ -672,13 +763,12 @@ class general_transform_program final : public bound_program
 {
 public:
 	general_transform_program(span<const operand_descriptor> outputs, span<const operand_descriptor> inputs,
-		transform_kind kind, const ops::axis_list &axes, ops::fourier_normalization normalization)
+		transform_kind kind, span<const std::size_t> axes, ops::fourier_normalization normalization)
 		: bound_program(outputs, inputs), m_kind(kind), m_normalization(abi_normalization(normalization))
 	{
-		const std::ptrdiff_t rank = static_cast<std::ptrdiff_t>(inputs[0].get_layout().get_rank());
-		for (const std::ptrdiff_t a : axes)
+		for (std::size_t i = 0; i < axes.size(); ++i)
 		{
-			m_axes.push_back(static_cast<int32_t>(a < 0 ? a + rank : a));
+			m_axes.push_back(static_cast<int32_t>(axes[i]));
 		}
 	}
 	void enqueue(span<const std::shared_ptr<buffer>> outputs, span<const std::shared_ptr<const buffer>> inputs,
@@ -710,10 +800,10 @@ private:
 };
 
 bool general_transform_suitable(span<const operand_descriptor> outputs, span<const operand_descriptor> inputs,
-	command_queue &queue, numerical_type in_type, numerical_type out_type, const ops::axis_list &axes)
+	command_queue &queue, numerical_type in_type, numerical_type out_type, span<const std::size_t> axes)
 {
-	return on_hip_queue(queue) && outputs.size() == 1 && inputs.size() == 1 && all_device_accessible(outputs) &&
-		all_device_accessible(inputs) && inputs[0].get_data_type() == in_type &&
+	return on_hip_queue(queue) && outputs.size() == 1 && inputs.size() == 1 && all_device_accessible(outputs, queue) &&
+		all_device_accessible(inputs, queue) && inputs[0].get_data_type() == in_type &&
 		outputs[0].get_data_type() == out_type && !axes.empty() &&
 		inputs[0].get_layout().get_rank() <= REXLIB_HIP_MAX_RANK &&
 		inputs[0].get_layout().get_rank() == outputs[0].get_layout().get_rank();
@@ -729,7 +819,7 @@ public:
 	{
 		const auto &f = operation_cast<Operation>(op);
 		return general_transform_suitable(outputs, inputs, queue, numerical_type::complex_float32,
-				   numerical_type::complex_float32, f.get_axes())
+				   numerical_type::complex_float32, f.get_parameters().get_axes())
 			? backend_priority::optimal
 			: backend_priority::unsupported;
 	}
@@ -737,7 +827,7 @@ public:
 		span<const operand_descriptor> inputs, command_queue &, program_cache *) const override
 	{
 		const auto &f = operation_cast<Operation>(op);
-		return std::make_shared<general_transform_program>(outputs, inputs, Kind, f.get_axes(),
+		return std::make_shared<general_transform_program>(outputs, inputs, Kind, f.get_parameters().get_axes(),
 			f.get_normalization());
 	}
 };
@@ -765,7 +855,7 @@ public:
 		const auto &f = operation_cast<ops::rfft_operation>(op);
 		if (fast(f, outputs, inputs, queue) ||
 			general_transform_suitable(outputs, inputs, queue, numerical_type::float32, numerical_type::complex_float32,
-				f.get_axes()))
+				f.get_parameters().get_axes()))
 		{
 			return backend_priority::optimal;
 		}
@@ -779,7 +869,8 @@ public:
 		{
 			return std::make_shared<rfft_program>(outputs, inputs);
 		}
-		return std::make_shared<general_transform_program>(outputs, inputs, transform_kind::rfft, f.get_axes(),
+		return std::make_shared<general_transform_program>(outputs, inputs, transform_kind::rfft,
+			f.get_parameters().get_axes(),
 			f.get_normalization());
 	}
 
@@ -788,16 +879,16 @@ private:
 	static bool fast(const ops::rfft_operation &f, span<const operand_descriptor> outputs,
 		span<const operand_descriptor> inputs, command_queue &queue)
 	{
-		if (!on_hip_queue(queue) || outputs.size() != 1 || inputs.size() != 1 || !all_device_accessible(outputs) ||
-			!all_device_accessible(inputs) || f.get_normalization() != ops::fourier_normalization::backward ||
-			!last_axes(f.get_axes(), inputs[0].get_layout().get_rank(), 2) ||
+		if (!on_hip_queue(queue) || outputs.size() != 1 || inputs.size() != 1 || !all_device_accessible(outputs, queue) ||
+			!all_device_accessible(inputs, queue) || f.get_normalization() != ops::fourier_normalization::backward ||
+			!last_axes(f.get_parameters().get_axes(), inputs[0].get_layout().get_rank(), 2) ||
 			outputs[0].get_data_type() != numerical_type::complex_float32 ||
 			(inputs[0].get_data_type() != numerical_type::float32 &&
 				inputs[0].get_data_type() != numerical_type::float16))
 		{
 			return false;
 		}
-		const auto &e = inputs[0].get_layout().get_extents();
+		const std::vector<std::size_t> e = extents_of(inputs[0]);
 		return e[e.size() - 1] == e[e.size() - 2] && smooth_length(e.back());
 	}
 };
@@ -832,7 +923,7 @@ public:
 		const auto &f = operation_cast<ops::irfft_operation>(op);
 		if (fast(f, outputs, inputs, queue) ||
 			general_transform_suitable(outputs, inputs, queue, numerical_type::complex_float32, numerical_type::float32,
-				f.get_axes()))
+				f.get_parameters().get_axes()))
 		{
 			return backend_priority::optimal;
 		}
@@ -844,10 +935,10 @@ public:
 		const auto &f = operation_cast<ops::irfft_operation>(op);
 		if (fast(f, outputs, inputs, queue))
 		{
-			return std::make_shared<irfft_program>(outputs, inputs, static_cast<int>(f.get_axes().size()));
+			return std::make_shared<irfft_program>(outputs, inputs, static_cast<int>(f.get_parameters().get_axes().size()));
 		}
-		return std::make_shared<general_transform_program>(outputs, inputs, transform_kind::irfft, f.get_axes(),
-			f.get_normalization());
+		return std::make_shared<general_transform_program>(outputs, inputs, transform_kind::irfft,
+			f.get_parameters().get_axes(), f.get_normalization());
 	}
 
 private:
@@ -855,20 +946,21 @@ private:
 	static bool fast(const ops::irfft_operation &f, span<const operand_descriptor> outputs,
 		span<const operand_descriptor> inputs, command_queue &queue)
 	{
-		if (!on_hip_queue(queue) || outputs.size() != 1 || inputs.size() != 1 || !all_device_accessible(outputs) ||
-			!all_device_accessible(inputs) || f.get_normalization() != ops::fourier_normalization::backward ||
+		if (!on_hip_queue(queue) || outputs.size() != 1 || inputs.size() != 1 || !all_device_accessible(outputs, queue) ||
+			!all_device_accessible(inputs, queue) || f.get_normalization() != ops::fourier_normalization::backward ||
 			outputs[0].get_data_type() != numerical_type::float32 ||
 			inputs[0].get_data_type() != numerical_type::complex_float32)
 		{
 			return false;
 		}
 		const std::size_t rank = inputs[0].get_layout().get_rank();
-		const std::size_t dims = last_axes(f.get_axes(), rank, 2) ? 2 : (last_axes(f.get_axes(), rank, 3) ? 3 : 0);
+		const span<const std::size_t> axes = f.get_parameters().get_axes();
+		const std::size_t dims = last_axes(axes, rank, 2) ? 2 : (last_axes(axes, rank, 3) ? 3 : 0);
 		if (dims == 0)
 		{
 			return false;
 		}
-		const auto &e = outputs[0].get_layout().get_extents();
+		const std::vector<std::size_t> e = extents_of(outputs[0]);
 		for (std::size_t a = rank - dims; a < rank; ++a)
 		{
 			if (e[a] != e.back())
@@ -883,11 +975,54 @@ private:
 // Fill of an operand in device memory with the operation's value, by a fill
 // kernel (not hipMemsetAsync, so that a captured graph holds kernel nodes
 // only), any strides.
+// The fill value as real and imaginary parts; false for a scalar type the
+// fill program does not take.
+bool fill_value_parts(const scalar_value &v, double &re, double &im)
+{
+	re = 0.0;
+	im = 0.0;
+	if (const float *x = v.get_if<float>())
+	{
+		re = *x;
+	}
+	else if (const double *x = v.get_if<double>())
+	{
+		re = *x;
+	}
+	else if (const std::complex<float> *x = v.get_if<std::complex<float>>())
+	{
+		re = x->real();
+		im = x->imag();
+	}
+	else if (const std::complex<double> *x = v.get_if<std::complex<double>>())
+	{
+		re = x->real();
+		im = x->imag();
+	}
+	else if (const bool *x = v.get_if<bool>())
+	{
+		re = *x ? 1.0 : 0.0;
+	}
+	else if (const std::int32_t *x = v.get_if<std::int32_t>())
+	{
+		re = static_cast<double>(*x);
+	}
+	else if (const std::int64_t *x = v.get_if<std::int64_t>())
+	{
+		re = static_cast<double>(*x);
+	}
+	else
+	{
+		return false;
+	}
+	return true;
+}
+
 class fill_program final : public bound_program
 {
 public:
-	fill_program(span<const operand_descriptor> outputs, span<const operand_descriptor> inputs, double value)
-		: bound_program(outputs, inputs), m_value(value)
+	fill_program(span<const operand_descriptor> outputs, span<const operand_descriptor> inputs, double re, double im)
+		: bound_program(outputs, inputs), m_re(re), m_im(im)
 	{
 	}
 	void enqueue(span<const std::shared_ptr<buffer>> outputs, span<const std::shared_ptr<const buffer>> inputs,
@@ -895,11 +1030,12 @@ public:
 	{
 		std::vector<rexlib_hip_array> out;
 		arrays(outputs, inputs, out);
-		abi_check(rexlib_hip_fill(&out[0], m_value, 0.0, stream));
+		abi_check(rexlib_hip_fill(&out[0], m_re, m_im, stream));
 	}
 
 private:
-	double m_value;
+	double m_re;
+	double m_im;
 };
 
 bool fill_type(numerical_type t)
@@ -912,11 +1048,13 @@ class fill_builder final : public program_builder
 {
 public:
 	operation_id get_operation_id() const noexcept override { return operation_id::of<ops::fill_operation>(); }
-	backend_priority get_suitability(const operation &, span<const operand_descriptor> outputs,
+	backend_priority get_suitability(const operation &op, span<const operand_descriptor> outputs,
 		span<const operand_descriptor> inputs, command_queue &queue) const override
 	{
-		if (!on_hip_queue(queue) || outputs.size() != 1 || !inputs.empty() || !all_device_accessible(outputs) ||
-			!fill_type(outputs[0].get_data_type()) || outputs[0].get_layout().get_rank() > REXLIB_HIP_MAX_RANK)
+		double re = 0.0, im = 0.0;
+		if (!on_hip_queue(queue) || outputs.size() != 1 || !inputs.empty() || !all_device_accessible(outputs, queue) ||
+			!fill_type(outputs[0].get_data_type()) || outputs[0].get_layout().get_rank() > REXLIB_HIP_MAX_RANK ||
+			!fill_value_parts(operation_cast<ops::fill_operation>(op).get_fill_value(), re, im))
 		{
 			return backend_priority::unsupported;
 		}
@@ -925,7 +1063,12 @@ public:
 	std::shared_ptr<program> build(const operation &op, span<const operand_descriptor> outputs,
 		span<const operand_descriptor> inputs, command_queue &, program_cache *) const override
 	{
-		return std::make_shared<fill_program>(outputs, inputs, operation_cast<ops::fill_operation>(op).get_value());
+		double re = 0.0, im = 0.0;
+		if (!fill_value_parts(operation_cast<ops::fill_operation>(op).get_fill_value(), re, im))
+		{
+			throw unsupported_operation_error("hip: fill values of this scalar type are not supported");
+		}
+		return std::make_shared<fill_program>(outputs, inputs, re, im);
 	}
 };
 
@@ -979,8 +1122,8 @@ public:
 		span<const operand_descriptor> inputs, command_queue &queue) const override
 	{
 		const std::size_t arity = Kind == elementwise_kind::where ? 3 : 2;
-		if (!on_hip_queue(queue) || outputs.size() != 1 || inputs.size() != arity || !all_device_accessible(outputs) ||
-			!all_device_accessible(inputs))
+		if (!on_hip_queue(queue) || outputs.size() != 1 || inputs.size() != arity || !all_device_accessible(outputs, queue) ||
+			!all_device_accessible(inputs, queue))
 		{
 			return backend_priority::unsupported;
 		}
@@ -1020,7 +1163,7 @@ public:
 			m_outputs[0].get_layout().get_offset() * static_cast<std::ptrdiff_t>(element);
 		const char *src = static_cast<const char *>(operand_address(*inputs[0])) +
 			m_inputs[0].get_layout().get_offset() * static_cast<std::ptrdiff_t>(element);
-		hip_check(hipMemcpyAsync(dst, src, m_outputs[0].get_layout().get_element_count() * element, hipMemcpyDefault,
+		hip_check(hipMemcpyAsync(dst, src, m_outputs[0].get_layout().compute_element_count() * element, hipMemcpyDefault,
 					  stream),
 			"hipMemcpyAsync");
 	}
@@ -1035,10 +1178,10 @@ public:
 	{
 		if (!on_hip_queue(queue) || outputs.size() != 1 || inputs.size() != 1 ||
 			outputs[0].get_memory_resource() == nullptr || inputs[0].get_memory_resource() == nullptr ||
-			(!device_accessible(outputs[0]) && !device_accessible(inputs[0])) ||
+			(!device_accessible(outputs[0], queue) && !device_accessible(inputs[0], queue)) ||
 			outputs[0].get_data_type() != inputs[0].get_data_type() ||
-			outputs[0].get_layout().get_extents() != inputs[0].get_layout().get_extents() ||
-			!outputs[0].get_layout().is_contiguous() || !inputs[0].get_layout().is_contiguous())
+			extents_of(outputs[0]) != extents_of(inputs[0]) || !contiguous(outputs[0].get_layout()) ||
+			!contiguous(inputs[0].get_layout()))
 		{
 			return backend_priority::unsupported;
 		}

@@ -31,7 +31,12 @@ public:
 	const array_descriptor &get_descriptor() const noexcept { return m_descriptor; }
 	const strided_layout &get_layout() const noexcept { return m_descriptor.get_layout(); }
 	numerical_type get_data_type() const noexcept { return m_descriptor.get_data_type(); }
-	const std::vector<std::size_t> &get_extents() const noexcept { return get_layout().get_extents(); }
+	std::vector<std::size_t> get_extents() const
+	{
+		std::vector<std::size_t> e;
+		get_layout().get_extents(e);
+		return e;
+	}
 	const memory_resource *get_memory_resource() const noexcept
 	{
 		return m_buffer ? &m_buffer->get_memory_resource() : nullptr;
@@ -137,12 +142,12 @@ namespace ops
 class REXLIB_API fill_operation final : public basic_operation<fill_operation>
 {
 public:
-	explicit fill_operation(double value) noexcept : m_value(value) {}
-	double get_value() const noexcept { return m_value; }
+	explicit fill_operation(const scalar_value &fill_value) noexcept : m_fill_value(fill_value) {}
+	const scalar_value &get_fill_value() const noexcept { return m_fill_value; }
 	const char *get_name() const noexcept override { return "fill"; }
 
 private:
-	double m_value;
+	scalar_value m_fill_value;
 };
 
 // Copies one operand into another of the same extents and type (src/functional/transfer.cpp:60-78).

@@ -29,8 +29,10 @@
 // MSVC branches of the ABI record.
 #pragma once
 
+#include <complex>
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -154,6 +156,48 @@ enum class numerical_type
 REXLIB_API std::size_t get_size(numerical_type type) noexcept;
 REXLIB_API const char *to_string(numerical_type type) noexcept;
 
+// A scalar of any numerical type, held by value
+// (core/numerical/scalar_value.hpp:34-77): constructed from a value of a
+// supported type, read back with get_if<T>() for the type it holds.
+template <typename T>
+struct scalar_type_of;
+template <> struct scalar_type_of<bool> { static constexpr numerical_type value = numerical_type::boolean; };
+template <> struct scalar_type_of<std::int8_t> { static constexpr numerical_type value = numerical_type::int8; };
+template <> struct scalar_type_of<std::uint8_t> { static constexpr numerical_type value = numerical_type::uint8; };
+template <> struct scalar_type_of<std::int16_t> { static constexpr numerical_type value = numerical_type::int16; };
+template <> struct scalar_type_of<std::uint16_t> { static constexpr numerical_type value = numerical_type::uint16; };
+template <> struct scalar_type_of<std::int32_t> { static constexpr numerical_type value = numerical_type::int32; };
+template <> struct scalar_type_of<std::uint32_t> { static constexpr numerical_type value = numerical_type::uint32; };
+template <> struct scalar_type_of<std::int64_t> { static constexpr numerical_type value = numerical_type::int64; };
+template <> struct scalar_type_of<std::uint64_t> { static constexpr numerical_type value = numerical_type::uint64; };
+template <> struct scalar_type_of<float> { static constexpr numerical_type value = numerical_type::float32; };
+template <> struct scalar_type_of<double> { static constexpr numerical_type value = numerical_type::float64; };
+template <> struct scalar_type_of<std::complex<float>> {
+	static constexpr numerical_type value = numerical_type::complex_float32; };
+template <> struct scalar_type_of<std::complex<double>> {
+	static constexpr numerical_type value = numerical_type::complex_float64; };
+
+class scalar_value
+{
+public:
+	template <typename T>
+	scalar_value(const T &value) noexcept : m_data_type(scalar_type_of<T>::value)
+	{
+		static_assert(sizeof(T) <= sizeof(m_storage), "scalar too large");
+		std::memcpy(m_storage, &value, sizeof(T));
+	}
+	template <typename T>
+	const T *get_if() const noexcept
+	{
+		return m_data_type == scalar_type_of<T>::value ? reinterpret_cast<const T *>(m_storage) : nullptr;
+	}
+	numerical_type get_data_type() const noexcept { return m_data_type; }
+
+private:
+	alignas(std::complex<double>) unsigned char m_storage[sizeof(std::complex<double>)];
+	numerical_type m_data_type;
+};
+
 enum class memory_resource_kind
 {
 	unified,
@@ -205,16 +249,13 @@ public:
 	// Row-major contiguous layout of the given extents.
 	static strided_layout make_contiguous(std::vector<std::size_t> extents);
 
+	// Accessors as strided_layout.hpp:48-88 declares them (extents and
+	// strides through out-parameters).
 	std::size_t get_rank() const noexcept { return m_extents.size(); }
-	const std::vector<std::size_t> &get_extents() const noexcept { return m_extents; }
-	const std::vector<std::ptrdiff_t> &get_strides() const noexcept { return m_strides; }
 	void get_extents(std::vector<std::size_t> &out) const { out = m_extents; }
 	void get_strides(std::vector<std::ptrdiff_t> &out) const { out = m_strides; }
 	std::ptrdiff_t get_offset() const noexcept { return m_offset; }
-	std::size_t get_element_count() const noexcept;
-	bool is_contiguous() const noexcept;
-	// Elements spanned from the offset to the last addressed element, plus one.
-	std::size_t get_span() const noexcept;
+	std::size_t compute_element_count() const noexcept;
 
 	friend bool operator==(const strided_layout &a, const strided_layout &b) noexcept
 	{
@@ -413,11 +454,38 @@ public:
 	virtual std::shared_ptr<command_queue> create_command_queue() const = 0;
 };
 
-struct device_properties
+// What a backend reports of a device (hardware/device_properties.hpp:26-153):
+// set by the backend, read by the caller; out-of-line accessors.
+enum class device_type
 {
-	std::string name;
-	std::size_t total_memory = 0;
-	std::size_t optimal_alignment = 256;
+	unknown = -1,
+	cpu,
+	gpu,
+	integrated_gpu,
+};
+
+class REXLIB_API device_properties
+{
+public:
+	device_properties();
+	~device_properties();
+	device_properties(const device_properties &other);
+	device_properties &operator=(const device_properties &other);
+
+	void set_type(device_type type);
+	device_type get_type() const noexcept;
+	void set_name(const std::string &name);
+	const std::string &get_name() const noexcept;
+	void set_physical_location(const std::string &location);
+	const std::string &get_physical_location() const noexcept;
+	void set_total_memory_bytes(std::size_t bytes);
+	std::size_t get_total_memory_bytes() const noexcept;
+	void set_optimal_data_alignment(std::size_t alignment);
+	std::size_t get_optimal_data_alignment() const noexcept;
+
+private:
+	struct implementation;
+	std::unique_ptr<implementation> m_implementation;
 };
 
 class REXLIB_API device_backend

@@ -1,0 +1,5 @@
+// include/rexlib/core/dispatch/program_service.hpp of rexlib 0.1.0: the declarations the plugin
+// uses are restated in rexlib/core.hpp.
+#pragma once
+
+#include "../../core.hpp"

@@ -1,0 +1,5 @@
+// include/rexlib/core/exceptions/unsupported_operation_error.hpp of rexlib 0.1.0: the declarations the plugin
+// uses are restated in rexlib/core.hpp.
+#pragma once
+
+#include "../../core.hpp"

@@ -1,0 +1,5 @@
+// include/rexlib/ops/fourier/ifft_operation.hpp of rexlib 0.1.0: the declarations the plugin
+// uses are restated in rexlib/fourier.hpp.
+#pragma once
+
+#include "../../fourier.hpp"

@@ -175,7 +175,7 @@ std::vector<char> read_file(const std::string &path)
 void write_file(const std::string &path, const array &host)
 {
 	std::ofstream f(path, std::ios::binary);
-	const std::size_t bytes = host.get_layout().get_element_count() * get_size(host.get_data_type());
+	const std::size_t bytes = host.get_layout().compute_element_count() * get_size(host.get_data_type());
 	f.write(static_cast<const char *>(host.get_buffer()->get_host_ptr()), static_cast<std::streamsize>(bytes));
 }
 
@@ -183,7 +183,7 @@ array host_from_file(const std::string &path, std::vector<std::size_t> extents, 
 {
 	array a = make_host_array(extents, type);
 	const std::vector<char> bytes = read_file(path);
-	const std::size_t need = a.get_layout().get_element_count() * get_size(type);
+	const std::size_t need = a.get_layout().compute_element_count() * get_size(type);
 	if (bytes.size() != need)
 	{
 		throw std::runtime_error(path + ": " + std::to_string(bytes.size()) + " bytes, expected " +
@@ -239,11 +239,13 @@ int run(const std::string &path, const std::string &dir)
 	// The two halves of the batch are views of the same device arrays.
 	auto part = [&](const array &a, std::size_t begin, std::size_t end) {
 		const strided_layout &l = a.get_layout();
-		std::vector<std::size_t> e = l.get_extents();
+		std::vector<std::size_t> e;
+		std::vector<std::ptrdiff_t> s;
+		l.get_extents(e);
+		l.get_strides(s);
 		e[0] = end - begin;
-		return a.view(array_descriptor(
-			strided_layout(e, l.get_strides(), l.get_offset() + static_cast<std::ptrdiff_t>(begin) * l.get_strides()[0]),
-			a.get_data_type()));
+		const std::ptrdiff_t offset = l.get_offset() + static_cast<std::ptrdiff_t>(begin) * s[0];
+		return a.view(array_descriptor(strided_layout(e, s, offset), a.get_data_type()));
 	};
 	const std::size_t halves[3] = {0, split, count};
 	for (int k = 0; k < 2; ++k)
@@ -266,7 +268,7 @@ int run(const std::string &path, const std::string &dir)
 	{
 		const float *d = static_cast<const float *>(den_host.get_buffer()->get_host_ptr());
 		float *c = static_cast<float *>(den_c_host.get_buffer()->get_host_ptr());
-		for (std::size_t i = 0; i < den_host.get_layout().get_element_count(); ++i)
+		for (std::size_t i = 0; i < den_host.get_layout().compute_element_count(); ++i)
 		{
 			c[2 * i] = d[i] > 0.0f ? d[i] : 1.0f;
 			c[2 * i + 1] = 0.0f;
@@ -284,7 +286,7 @@ int run(const std::string &path, const std::string &dir)
 	std::cout << "{\"run\": \"ok\", \"device\": "
 			  << quote(catalog.get_service<device_service>()->create_device_session(device_index("hip", 0))
 						   ->get_properties()
-						   .name)
+						   .get_name())
 			  << "}" << std::endl;
 	return 0;
 }
