@@ -43,7 +43,8 @@ typedef enum rexlib_hip_dtype {
 	REXLIB_HIP_COMPLEX128 = 3,
 	REXLIB_HIP_FLOAT16 = 4,
 	REXLIB_HIP_BFLOAT16 = 5,
-	REXLIB_HIP_BOOL = 6 /* one byte, 0 or 1 */
+	REXLIB_HIP_BOOL = 6, /* one byte, 0 or 1 */
+	REXLIB_HIP_UINT64 = 7
 } rexlib_hip_dtype;
 
 /* A strided view of device memory: the address of element (0, ..., 0),
@@ -306,9 +307,35 @@ int rexlib_hip_gridding_correct(const rexlib_hip_array *volume, const rexlib_hip
 /* float32 / complex64; float64 operands are UNSUPPORTED (the reference's    */
 /* CPU backend keeps them). out may alias in.                                 */
 /* Replace the programs behind ops::fft_operation, ifft_operation,            */
-/* rfft_operation and irfft_operation (include/rexlib/ops/fourier/*.hpp; CPU */
+/* rfft_operation and irfft_operation (include/rexlib/ops/fourier/; CPU     */
 /* src/backends/cpu/kernels/pocketfft_fourier_transform.cpp:74-163).          */
 /* ------------------------------------------------------------------------- */
+
+/* ------------------------------------------------------------------------- */
+/* Background standardization of images: replaces the program behind          */
+/* em::standardize_background_operation                                       */
+/* (include/rexlib/em/ops/preprocessing/standardize_background_operation.hpp; */
+/* CPU src/backends/cpu/builders/em/preprocessing/).                          */
+/* ------------------------------------------------------------------------- */
+
+/* An image is the value's part along the axes whose bits are set in
+ * image_axes (bit a for axis a); the other axes are batch axes. Per image:
+ * mean and population deviation, in double, of the finite background
+ * elements (squared distance from the centre, index n / 2 along each image
+ * axis, above radius^2); again over those whose value standardized with the
+ * first statistics lies in [lower_bound, upper_bound]; then result =
+ * (value - mean) * (1 / deviation) in the element type, and elements not
+ * finite or outside the bounds take the standard normal draw of their
+ * counter: the Box-Muller pair of the Philox4x32-10 block (c / 2, 0, 0, 0)
+ * keyed by the image's key, cosine branch for even c, sine for odd. The
+ * counter is the element's row-major position among the elements sharing
+ * its key. value and result: float32 or float64, same extents (any
+ * strides; result may alias value only element for element); key: uint64,
+ * broadcasting against the batch axes from the right. radius >= 0,
+ * lower_bound <= 0 <= upper_bound. */
+int rexlib_hip_standardize_background(const rexlib_hip_array *value, const rexlib_hip_array *key,
+	uint32_t image_axes, double radius, double lower_bound, double upper_bound, const rexlib_hip_array *result,
+	void *stream);
 
 /* Complex-to-complex DFT over `axes`: forward (exp(-i...)) or inverse. */
 int rexlib_hip_fft(const rexlib_hip_array *in, const rexlib_hip_array *out, const int32_t *axes,

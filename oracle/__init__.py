@@ -23,6 +23,12 @@ Contents:
 * Closed forms for the steps the reference has no function for: the weight
   quotient (``tests/integration/src/em/functional/test_cpu_reconstruction.cpp:457-464``),
   gridding correction, per-shell sums and FSC.
+* ``standardize_background``: the background standardization of
+  ``include/rexlib/em/ops/preprocessing/standardize_background_operation.hpp:15-76``
+  (``src/backends/cpu/builders/em/preprocessing/``), with its draws
+  (``philox4x32_10``, ``standard_normal``: ``src/backends/cpu/kernels/philox.inl``,
+  ``standard_normal.inl``), pinned by Random123's known answers and the
+  reference kernel tests' own numbers.
 """
 from __future__ import annotations
 
@@ -455,4 +461,113 @@ def compensation_features(den):
                     s += np.roll(d, (-dz, -dy, -dx), axis=(0, 1, 2))
                 out[..., 2 + b] = np.log1p(s[:, :, :h] / 7.0)
                 b += 1
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Background standardization
+# ---------------------------------------------------------------------------
+
+_PHILOX_M = (0xD2511F53, 0xCD9E8D57)
+_PHILOX_W = (0x9E3779B9, 0xBB67AE85)
+_MASK32 = 0xFFFFFFFF
+
+
+def philox4x32_10(counter, key):
+    """Philox4x32-10 of one block (src/backends/cpu/kernels/philox.inl:14-113;
+    Salmon et al., SC 2011): (c0, c1, c2, c3), (k0, k1) -> four words."""
+    c = [int(x) & _MASK32 for x in counter]
+    k = [int(x) & _MASK32 for x in key]
+    for r in range(10):
+        if r:
+            k = [(k[0] + _PHILOX_W[0]) & _MASK32, (k[1] + _PHILOX_W[1]) & _MASK32]
+        p0 = _PHILOX_M[0] * c[0]
+        p1 = _PHILOX_M[1] * c[2]
+        c = [((p1 >> 32) ^ c[1] ^ k[0]) & _MASK32, p1 & _MASK32, ((p0 >> 32) ^ c[3] ^ k[1]) & _MASK32, p0 & _MASK32]
+    return tuple(c)
+
+
+def uniform_open_closed(bits: int) -> float:
+    """Top 53 bits plus one, times 2^-53: in (0, 1] (standard_normal.inl:20-25)."""
+    return float((int(bits) >> 11) + 1) * (1.0 / 9007199254740992.0)
+
+
+def standard_normal_pair(key: int, block: int):
+    """Box-Muller of the block (block, 0, 0, 0) keyed by key (standard_normal.inl:27-44)."""
+    key = int(key) & 0xFFFFFFFFFFFFFFFF
+    block = int(block)
+    r = philox4x32_10((block & _MASK32, block >> 32, 0, 0), (key & _MASK32, key >> 32))
+    first = (r[1] << 32) | r[0]
+    second = (r[3] << 32) | r[2]
+    radius = math.sqrt(-2.0 * math.log(uniform_open_closed(first)))
+    angle = 6.283185307179586476925 * uniform_open_closed(second)
+    return radius * math.cos(angle), radius * math.sin(angle)
+
+
+def standard_normal(key: int, counter: int) -> float:
+    """Counters 2k and 2k + 1 take the pair of block k (standard_normal.inl:46-50)."""
+    return standard_normal_pair(key, counter // 2)[counter % 2]
+
+
+def background_moments(image, background, mean, scale, lower, upper):
+    """(mean, 1 / deviation) of the finite background elements whose value,
+    standardized with (mean, scale), lies within [lower, upper], in double;
+    (0, 1) without any (background_moments_kernel.inl:117-135, 206-213)."""
+    x = image.astype(np.float64)
+    with np.errstate(invalid="ignore", over="ignore"):
+        standardized = (x - mean) * scale
+        valid = background & np.isfinite(x) & (standardized >= lower) & (standardized <= upper)
+    count = int(valid.sum())
+    if count == 0:
+        return 0.0, 1.0
+    kept = x[valid]
+    m = float(kept.sum()) / count
+    variance = float((kept * kept).sum()) / count - m * m
+    return m, (1.0 / math.sqrt(variance) if variance > 0.0 else 1.0)
+
+
+def background_mask(shape, radius):
+    """Elements whose squared distance (in elements) from index n // 2 per
+    axis exceeds radius^2 (background_geometry.inl:46-55)."""
+    grids = np.meshgrid(*[np.arange(n) - n // 2 for n in shape], indexing="ij")
+    squared = sum(g.astype(np.int64) ** 2 for g in grids)
+    return squared.astype(np.float64) > float(radius) * float(radius)
+
+
+def standardize_background(value, key, radius, lower, upper, image_rank=2):
+    """The operation over trailing image axes, value (..., image), key
+    broadcasting against the batch axes; the counter of an element is its
+    row-major position among the elements sharing its key
+    (standardize_background_operation.hpp:41-56)."""
+    value = np.asarray(value)
+    dtype = value.dtype.type
+    batch_shape = value.shape[:value.ndim - image_rank]
+    image_shape = value.shape[value.ndim - image_rank:]
+    key = np.broadcast_to(np.asarray(key, dtype=np.uint64), batch_shape) if batch_shape else np.asarray(key, np.uint64)
+    key_full = np.asarray(key, dtype=np.uint64)
+    key_given = np.asarray(key, dtype=np.uint64)
+    background = background_mask(image_shape, radius)
+    image_elements = int(np.prod(image_shape)) if image_shape else 1
+    out = np.empty_like(value)
+    # Batch axes the key is shared along: its base (non-broadcast) strides are 0 there.
+    shared = [key_full.strides[a] == 0 if key_full.ndim else True for a in range(len(batch_shape))]
+    lower_t, upper_t = dtype(lower), dtype(upper)
+    for b in np.ndindex(*batch_shape) if batch_shape else [()]:
+        image = value[b]
+        m1, s1 = background_moments(image, background, 0.0, 1.0, -math.inf, math.inf)
+        m2, s2 = background_moments(image, background, m1, s1, lower, upper)
+        standardized = (image.astype(dtype) - dtype(m2)) * dtype(s2)
+        with np.errstate(invalid="ignore", over="ignore"):
+            kept = (standardized >= lower_t) & (standardized <= upper_t) & np.isfinite(standardized)
+        rank = 0
+        for a, i in enumerate(b):
+            if shared[a]:
+                rank = rank * batch_shape[a] + i
+        k = int(key_full[b]) if batch_shape else int(key_given)
+        result = standardized.copy()
+        flat_kept = kept.reshape(-1)
+        flat = result.reshape(-1)
+        for e in np.nonzero(~flat_kept)[0]:
+            flat[e] = dtype(standard_normal(k, rank * image_elements + int(e)))
+        out[b] = result
     return out

@@ -15,7 +15,7 @@ LIBRARY = os.path.join(HERE, "librexlib_hip.so")
 MAX_RANK = 8
 
 OK, INVALID_ARGUMENT, UNSUPPORTED, RUNTIME_ERROR = 0, 1, 2, 3
-FLOAT32, FLOAT64, COMPLEX64, COMPLEX128, FLOAT16, BFLOAT16, BOOL = 0, 1, 2, 3, 4, 5, 6
+FLOAT32, FLOAT64, COMPLEX64, COMPLEX128, FLOAT16, BFLOAT16, BOOL, UINT64 = 0, 1, 2, 3, 4, 5, 6, 7
 
 _DTYPES = {
     torch.float32: FLOAT32,
@@ -25,6 +25,7 @@ _DTYPES = {
     torch.float16: FLOAT16,
     torch.bfloat16: BFLOAT16,
     torch.bool: BOOL,
+    torch.uint64: UINT64,
 }
 
 
@@ -77,6 +78,7 @@ EXPORTS = [
     "rexlib_hip_divide",
     "rexlib_hip_greater",
     "rexlib_hip_where",
+    "rexlib_hip_standardize_background",
     "rexlib_hip_density_compensation",
     "rexlib_hip_compensation_features",
     "rexlib_hip_mlp_parameter_count",
@@ -135,6 +137,7 @@ def library() -> ctypes.CDLL:
     lib.rexlib_hip_greater.argtypes = [A, A, A, vp]
     lib.rexlib_hip_where.argtypes = [A, A, A, A, vp]
     d = ctypes.c_double
+    lib.rexlib_hip_standardize_background.argtypes = [A, A, ctypes.c_uint32, d, d, d, A, vp]
     lib.rexlib_hip_density_compensation.argtypes = [A, ctypes.c_int32, d, A, vp]
     lib.rexlib_hip_compensation_features.argtypes = [A, A, vp]
     lib.rexlib_hip_mlp_parameter_count.argtypes = []

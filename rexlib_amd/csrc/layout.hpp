@@ -60,6 +60,7 @@ inline const char *dtype_name(int dtype)
 	case REXLIB_HIP_FLOAT16: return "float16";
 	case REXLIB_HIP_BFLOAT16: return "bfloat16";
 	case REXLIB_HIP_BOOL: return "bool";
+	case REXLIB_HIP_UINT64: return "uint64";
 	default: return "unknown";
 	}
 }

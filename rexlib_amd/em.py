@@ -31,6 +31,7 @@ __all__ = [
     "insert_ctf_images",
     "normalize",
     "gridding_correct",
+    "standardize_background",
     "shell_sums",
     "fsc",
 ]
@@ -216,6 +217,38 @@ def gridding_correct(volume, window: GriddingWindow, out=None, crop: int | None 
         out = torch.empty(tuple(volume.shape[:-3]) + (m, m, m), dtype=volume.dtype, device=volume.device)
     N.check(N.library().rexlib_hip_gridding_correct(N.array(volume), window.native(), N.array(out),
                                                     ctx.handle()))
+    return out
+
+
+def standardize_background(value, key, radius: float, lower_bound: float, upper_bound: float, axes=(-2, -1),
+                           out=None, context: ExecutionContext | None = None):
+    """Standardize images against the statistics of their background
+    (include/rexlib/em/ops/preprocessing/standardize_background_operation.hpp).
+
+    An image is ``value``'s part along ``axes``; the other axes are batches.
+    The background of an image is the elements whose squared distance from
+    the centre (index n // 2 per axis) exceeds ``radius`` squared. Mean and
+    population deviation of its finite elements are taken twice, the second
+    time over the values that the first statistics standardize within
+    [lower_bound, upper_bound]; the result is (value - mean) / deviation, and
+    elements not finite or outside the bounds take standard normal draws of
+    the Philox4x32-10 stream of their ``key`` (uint64, or int64 read as its
+    bits, broadcasting against the batch axes)."""
+    ctx = _context(context)
+    rank = value.dim()
+    mask = 0
+    for a in axes:
+        a = a + rank if a < 0 else a
+        if not 0 <= a < rank or mask & (1 << a):
+            raise ValueError("standardize_background: axes must be distinct axes of the value")
+        mask |= 1 << a
+    if key.dtype == torch.int64:
+        key = key.view(torch.uint64)
+    if out is None:
+        out = torch.empty_like(value)
+    N.check(N.library().rexlib_hip_standardize_background(N.array(value), N.array(key), mask, float(radius),
+                                                          float(lower_bound), float(upper_bound), N.array(out),
+                                                          ctx.handle()))
     return out
 
 

@@ -1199,6 +1199,43 @@ double ctf_optics::get_wavelength() const noexcept
 	return 12.2643247 / std::sqrt(volts * (1.0 + 0.978466e-6 * volts));
 }
 
+standardize_background_operation::standardize_background_operation(ops::axis_list axes, double radius,
+	double lower_bound, double upper_bound)
+	: m_axes(axes), m_radius(radius), m_lower_bound(lower_bound), m_upper_bound(upper_bound)
+{
+	for (std::size_t i = 0; i < axes.size(); ++i)
+	{
+		for (std::size_t j = i + 1; j < axes.size(); ++j)
+		{
+			if (axes[i] == axes[j])
+			{
+				throw std::invalid_argument("standardize_background: an axis is repeated");
+			}
+		}
+	}
+	if (!(radius >= 0.0))
+	{
+		throw std::invalid_argument("standardize_background: the radius must be at least 0");
+	}
+	if (!(lower_bound <= 0.0 && 0.0 <= upper_bound))
+	{
+		throw std::invalid_argument("standardize_background: the bounds must lie on either side of 0");
+	}
+}
+
+array standardize_background(const_array_ref value, const_array_ref key, ops::axis_list axes, double radius,
+	double lower_bound, double upper_bound, const execution_context &context, array *out)
+{
+	check_context(context);
+	const standardize_background_operation op(std::move(axes), radius, lower_bound, upper_bound);
+	array local;
+	array &target = out ? *out : local;
+	const array_descriptor planned(strided_layout::make_contiguous(value->get_extents()), value->get_data_type());
+	const const_array_ref inputs[2] = {value, key};
+	context.get_dispatcher()->dispatch(op, {&target, 1}, {&planned, 1}, {inputs, 2}, context.get_device_context());
+	return target;
+}
+
 insert_ctf_slices_operation::insert_ctf_slices_operation(const gridding_window &window, const ctf_optics &optics,
 	double pixel_size)
 	: m_window(window), m_optics(optics), m_pixel_size(pixel_size)

@@ -12,7 +12,7 @@
 //   - program builders for insert_slices, insert_weighted_slices,
 //     insert_ctf_slices, fft, ifft, rfft and irfft (any axes; the square
 //     images and cubic grids of the path take dedicated kernels), fill, copy,
-//     divide, greater and where, which run the kernels of
+//     divide, greater, where and standardize_background, which run the kernels of
 //     librexlib_hip.so through its C ABI (include/rexlib_hip.h).
 //
 // Queue semantics follow plugins/cuda/src/hardware/cuda_command_queue.cpp:102-215:
@@ -50,6 +50,7 @@
 #include <rexlib/em/ops/reconstruction/insert_ctf_slices_operation.hpp>
 #include <rexlib/em/ops/reconstruction/insert_slices_operation.hpp>
 #include <rexlib/em/ops/reconstruction/insert_weighted_slices_operation.hpp>
+#include <rexlib/em/ops/preprocessing/standardize_background_operation.hpp>
 #include <rexlib/ops/arithmetic/divide_operation.hpp>
 #include <rexlib/ops/assignment/copy_operation.hpp>
 #include <rexlib/ops/assignment/fill_operation.hpp>
@@ -475,6 +476,8 @@ int abi_type(numerical_type t)
 		return REXLIB_HIP_FLOAT16;
 	case numerical_type::boolean:
 		return REXLIB_HIP_BOOL;
+	case numerical_type::uint64:
+		return REXLIB_HIP_UINT64;
 	default:
 		throw unsupported_operation_error(std::string("hip: no element type ") + to_string(t));
 	}
@@ -1194,6 +1197,84 @@ public:
 	}
 };
 
+// Background standardization over any axes of float32 / float64 images.
+class standardize_program final : public bound_program
+{
+public:
+	standardize_program(span<const operand_descriptor> outputs, span<const operand_descriptor> inputs, uint32_t axes,
+		double radius, double lower_bound, double upper_bound)
+		: bound_program(outputs, inputs), m_axes(axes), m_radius(radius), m_lower_bound(lower_bound),
+		  m_upper_bound(upper_bound)
+	{
+	}
+	void enqueue(span<const std::shared_ptr<buffer>> outputs, span<const std::shared_ptr<const buffer>> inputs,
+		hipStream_t stream) const override
+	{
+		std::vector<rexlib_hip_array> out;
+		const auto in = arrays(outputs, inputs, out);
+		abi_check(rexlib_hip_standardize_background(&in[0], &in[1], m_axes, m_radius, m_lower_bound, m_upper_bound,
+			&out[0], stream));
+	}
+
+private:
+	uint32_t m_axes;
+	double m_radius;
+	double m_lower_bound;
+	double m_upper_bound;
+};
+
+class standardize_builder final : public program_builder
+{
+public:
+	operation_id get_operation_id() const noexcept override
+	{
+		return operation_id::of<em::standardize_background_operation>();
+	}
+	backend_priority get_suitability(const operation &op, span<const operand_descriptor> outputs,
+		span<const operand_descriptor> inputs, command_queue &queue) const override
+	{
+		const auto &s = operation_cast<em::standardize_background_operation>(op);
+		if (!on_hip_queue(queue) || outputs.size() != 1 || inputs.size() != 2 || !all_device_accessible(outputs, queue) ||
+			!all_device_accessible(inputs, queue))
+		{
+			return backend_priority::unsupported;
+		}
+		const numerical_type t = inputs[0].get_data_type();
+		const std::size_t rank = inputs[0].get_layout().get_rank();
+		if ((t != numerical_type::float32 && t != numerical_type::float64) || outputs[0].get_data_type() != t ||
+			inputs[1].get_data_type() != numerical_type::uint64 || rank > REXLIB_HIP_MAX_RANK ||
+			axis_mask(s.get_parameters().get_axes(), rank) == 0)
+		{
+			return backend_priority::unsupported;
+		}
+		return backend_priority::optimal;
+	}
+	std::shared_ptr<program> build(const operation &op, span<const operand_descriptor> outputs,
+		span<const operand_descriptor> inputs, command_queue &, program_cache *) const override
+	{
+		const auto &s = operation_cast<em::standardize_background_operation>(op);
+		return std::make_shared<standardize_program>(outputs, inputs,
+			axis_mask(s.get_parameters().get_axes(), inputs[0].get_layout().get_rank()), s.get_radius(),
+			s.get_lower_bound(), s.get_upper_bound());
+	}
+
+private:
+	// The axes as a bit set (0 for an axis beyond the rank).
+	static uint32_t axis_mask(span<const std::size_t> axes, std::size_t rank)
+	{
+		uint32_t mask = 0;
+		for (std::size_t i = 0; i < axes.size(); ++i)
+		{
+			if (axes[i] >= rank)
+			{
+				return 0;
+			}
+			mask |= 1u << axes[i];
+		}
+		return mask;
+	}
+};
+
 // ---------------------------------------------------------------------------
 // The plugin
 // ---------------------------------------------------------------------------
@@ -1228,6 +1309,7 @@ public:
 			std::make_unique<elementwise_builder<ops::greater_operation, elementwise_kind::greater>>());
 		programs->register_builder(
 			std::make_unique<elementwise_builder<ops::where_operation, elementwise_kind::where>>());
+		programs->register_builder(std::make_unique<standardize_builder>());
 	}
 };
 

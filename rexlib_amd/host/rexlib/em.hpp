@@ -8,6 +8,10 @@
 //                              insert_weighted_slices_operation.hpp, insert_ctf_slices_operation.hpp:17-116
 //   insert_* functions         include/rexlib/em/functional/reconstruction.hpp:47-149,
 //                              src/em/functional/reconstruction.cpp:24-137
+//   standardize_background_operation
+//                              include/rexlib/em/ops/preprocessing/standardize_background_operation.hpp:15-118
+//   standardize_background_operation
+//                              include/rexlib/em/ops/preprocessing/standardize_background_operation.hpp:15-118
 //
 // The functions check parameters as the reference's operation constructors
 // do (std::invalid_argument) and leave operand checks to the program, which
@@ -16,7 +20,7 @@
 // numerator_out / denominator_out accumulates in place.
 #pragma once
 
-#include "array.hpp"
+#include "fourier.hpp"
 
 #include <utility>
 
@@ -108,6 +112,31 @@ private:
 	ctf_optics m_optics;
 	double m_pixel_size;
 };
+
+// Standardizes the images along `axes` against their background (a
+// result from a value and a uint64 key); throws std::invalid_argument when
+// an axis is repeated, the radius is negative or NaN, or the bounds do not
+// lie on either side of 0.
+class REXLIB_API standardize_background_operation final : public basic_operation<standardize_background_operation>
+{
+public:
+	using parameters_type = ops::axis_selection;
+	standardize_background_operation(ops::axis_list axes, double radius, double lower_bound, double upper_bound);
+	const ops::axis_selection &get_parameters() const noexcept { return m_axes; }
+	double get_radius() const noexcept { return m_radius; }
+	double get_lower_bound() const noexcept { return m_lower_bound; }
+	double get_upper_bound() const noexcept { return m_upper_bound; }
+	const char *get_name() const noexcept override { return "standardize_background"; }
+
+private:
+	ops::axis_selection m_axes;
+	double m_radius;
+	double m_lower_bound;
+	double m_upper_bound;
+};
+
+REXLIB_API array standardize_background(const_array_ref value, const_array_ref key, ops::axis_list axes, double radius,
+	double lower_bound, double upper_bound, const execution_context &context, array *out = nullptr);
 
 REXLIB_API std::pair<array, array> insert_slices(const_array_ref numerator, const_array_ref denominator,
 	const_array_ref slices, const_array_ref transforms, const_array_ref radial_weights,

@@ -279,6 +279,19 @@ int run(const std::string &path, const std::string &dir)
 	const array zero = full({1}, 0.0, numerical_type::complex_float32, ctx);
 	const array quotient = where(greater(den, threshold, ctx), divide(num, den_c, ctx), zero, ctx);
 	write_file(dir + "/quotient_out.bin", to_host(quotient, ctx));
+
+	// Background standardization of the images, a key per image.
+	array keys_host = make_host_array({count}, numerical_type::uint64);
+	{
+		std::uint64_t *k = static_cast<std::uint64_t *>(keys_host.get_buffer()->get_host_ptr());
+		for (std::size_t i = 0; i < count; ++i)
+		{
+			k[i] = 1000 + i;
+		}
+	}
+	const array standardized = em::standardize_background(images, to_device(keys_host, ctx), {1, 2},
+		static_cast<double>(n) / 2 - 2, -3.0, 3.0, ctx);
+	write_file(dir + "/standardized_out.bin", to_host(standardized, ctx));
 	write_file(dir + "/spectra_out.bin", to_host(spectra, ctx));
 	write_file(dir + "/num_out.bin", to_host(num, ctx));
 	write_file(dir + "/den_out.bin", to_host(den, ctx));

@@ -310,3 +310,63 @@ def test_one_select_each_way_makes_the_float_shell_estimate_exact():
     s = s - ((s > 0) & (4 * s * s - 4 * s + 1 > q4))
     exact = np.floor(np.sqrt(q.astype(np.float64)) + 0.5).astype(np.int64)
     assert np.array_equal(s, exact)
+
+
+# ---------------------------------------------------------------------------
+# Background standardization (the oracle's restatement, pinned)
+# ---------------------------------------------------------------------------
+
+def test_philox_reproduces_random123_known_answers():
+    # tests/unitary/src/backends/cpu/kernels/test_philox.cpp:13-38.
+    assert oracle.philox4x32_10((0, 0, 0, 0), (0, 0)) == (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)
+    assert oracle.philox4x32_10((0xffffffff,) * 4, (0xffffffff, 0xffffffff)) == (
+        0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)
+    assert oracle.philox4x32_10((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0)) == (
+        0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)
+
+
+def test_uniform_open_closed_spans_zero_to_one():
+    assert oracle.uniform_open_closed(0) == 2.0 ** -53
+    assert oracle.uniform_open_closed(0xFFFFFFFFFFFFFFFF) == 1.0
+
+
+def test_draws_are_the_pairs_in_counter_order_and_standard_normal():
+    key = 0x0123456789ABCDEF
+    first, second = oracle.standard_normal_pair(key, 21)
+    assert oracle.standard_normal(key, 42) == first
+    assert oracle.standard_normal(key, 43) == second
+    draws = np.array([oracle.standard_normal(key, c) for c in range(20000)])
+    assert abs(draws.mean()) < 0.03 and abs(draws.std() - 1.0) < 0.03
+
+
+def test_background_statistics_match_the_reference_kernel_test():
+    # test_standardize_background_kernel.cpp:65-86: a row of nine, background
+    # indices 0-2 and 6-8 (radius 1), bounds +-1. The first pass takes 23 in
+    # (mean 5.33, deviation 7.95); the second leaves it out: mean 1.8.
+    row = np.array([1, 3, 1, 0, 0, 0, 3, 1, 23], np.float64)
+    background = oracle.background_mask((9,), 1)
+    assert background.tolist() == [True, True, True, False, False, False, True, True, True]
+    mean, scale = oracle.background_moments(row, background, 0.0, 1.0, -np.inf, np.inf)
+    assert mean > 5 and abs(1.0 / scale - 7.951240294584376) < 1e-12
+    mean2, _ = oracle.background_moments(row, background, mean, scale, -1, 1)
+    assert mean2 == 1.8
+
+
+def test_background_mask_counts_lattice_points_outside_the_disk():
+    # 8 x 8 image, centre (4, 4), radius 2: 13 lattice points lie within
+    # distance 2 (the disk x^2 + y^2 <= 4), the other 51 are background.
+    mask = oracle.background_mask((8, 8), 2)
+    assert int((~mask).sum()) == 13 and not mask[4, 4] and mask[0, 0]
+
+
+def test_standardization_without_background_or_spread():
+    # No background element: mean 0, deviation 1; a constant background:
+    # the mean is subtracted and nothing scaled
+    # (standardize_background_operation.hpp:33-35).
+    image = np.arange(9, dtype=np.float64).reshape(3, 3)
+    out = oracle.standardize_background(image, 7, 10.0, -1e9, 1e9)
+    assert np.array_equal(out, image)
+    image = np.full((5, 5), 4.0)
+    image[2, 2] = 9.0
+    out = oracle.standardize_background(image, 7, 0.5, -1e9, 1e9)
+    assert out[2, 2] == 5.0 and out[0, 0] == 0.0

@@ -74,7 +74,7 @@ def test_registration_and_rules():
         "rexlib::em::insert_ctf_slices_operation", "rexlib::ops::rfft_operation", "rexlib::ops::fft_operation",
         "rexlib::ops::ifft_operation", "rexlib::ops::irfft_operation", "rexlib::ops::fill_operation",
         "rexlib::ops::copy_operation", "rexlib::ops::divide_operation", "rexlib::ops::greater_operation",
-        "rexlib::ops::where_operation"]
+        "rexlib::ops::where_operation", "rexlib::em::standardize_background_operation"]
     assert errors["foreign_queue"] == "unsupported_operation_error"
     assert errors["bad_pixel_size"] == errors["bad_optics"] == errors["bad_window"] == "invalid_argument"
     # lambda(300 kV), tests/unitary/src/em/ops/parameters/test_ctf_optics.cpp:40-53.

@@ -56,6 +56,11 @@ def test_reconstruction_through_the_plugin(tmp_path, profile):
     assert oracle.rel_l2(num, ref[0]) <= 1e-4
     assert oracle.rel_l2(den, ref[1]) <= 1e-4
 
+    standardized = np.fromfile(tmp_path / "standardized_out.bin", np.float32).reshape(count, n, n)
+    want_s = oracle.standardize_background(images, np.arange(1000, 1000 + count, dtype=np.uint64), n / 2 - 2,
+                                           -3.0, 3.0)
+    assert np.max(np.abs(standardized - want_s)) <= 1e-5
+
     quotient = np.fromfile(tmp_path / "quotient_out.bin", np.complex64).reshape(big, big, hb)
     want_q = oracle.normalize(num.astype(np.complex128), den.astype(np.float64), 1e-3)
     assert np.max(np.abs(quotient - want_q)) <= 1e-6 * np.max(np.abs(want_q))
