@@ -209,49 +209,236 @@ def precision_forms(args, stack, transforms, ctf, radial, reference, torch, em, 
     return out
 
 
-def learned_form(den, torch):
+def learned_form(den, transforms, big, torch):
     """Density compensation of the benchmark's weight volume: the 10-pass
-    iterative estimate against one forward pass of the perceptron (features +
-    MFMA forward over every voxel); untrained weights, since the timing does
-    not depend on their values."""
+    iterative estimate against the perceptron's weights (coverage table of the
+    transforms, features, MFMA forward over every voxel, exp / den). The
+    shipped model is used when present; the timing does not depend on the
+    parameter values."""
     from rexlib_amd import learned
-    model = learned.CompensationModel(seed=0)
+    model = learned.shipped_model() or learned.CompensationModel(seed=0)
     voxels = den.numel()
     iterative_ms = timed(torch, lambda: learned.iterative_weights(den), repeats=3)
-    learned_ms = timed(torch, lambda: model.weights(den), repeats=3)
-    rows = learned.features(den).reshape(-1, learned.FEATURES)
+    table_ms = timed(torch, lambda: learned.coverage_table(transforms, big), repeats=3)
+    table = learned.coverage_table(transforms, big)
+    learned_ms = timed(torch, lambda: model.weights(den, table), repeats=3)
+    rows = learned.features(den, table).reshape(-1, learned.FEATURES)
     forward_ms = timed(torch, lambda: model.predict(rows), repeats=3)
     flops = 2.0 * (10 * 64 + 64 * 64 + 64 * 64 + 64 * 1) * voxels
-    out = {"voxels": voxels, "iterative_ms": iterative_ms, "learned_ms": learned_ms,
+    out = {"voxels": voxels, "iterative_ms": iterative_ms, "coverage_table_ms": table_ms, "learned_ms": learned_ms,
            "mlp_forward_ms": forward_ms, "mlp_forward_tflops": flops / (forward_ms * 1e-3) / 1e12,
            "mlp_forward_frac_of_fp16_dense_peak": flops / (forward_ms * 1e-3) / 2.5e15}
     log(f"learned form: {out}")
     return out
 
 
-def batched_form(torch, count=16, seed=2000):
-    """Reconstructions of mixed box sizes ({64, 96, 128}) and particle counts
-    (log-uniform 500-4000): one HIP graph replay against the same runs one
-    after another."""
+def learned_config(torch, em, rexlib_amd, images=80000, launch=4096, seed=7000):
+    """The learned-weight form at its own shape: 256^3 from `images`
+    projections of 256^2 of a 400-blob volume (CTF defocus 1-3 um, SNR 0.1),
+    80% of the views von Mises-Fisher around the top view with kappa drawn
+    uniform in [2, 20], 20% uniform on SO(3). Two paths over the same images:
+
+    * plain: float32 images, one eager fused rfft2 + insertion call per
+      launch, the iterative density compensation (10 passes), irfft3 of
+      numerator * W, gridding correction;
+    * combined: the images staged in HBM as bfloat16, the zeroing and every
+      launch captured in one HIP graph and replayed, the shipped perceptron's
+      weights (``learned.Compensator``: judged once against the iterative
+      estimate, then reused), irfft3, gridding correction.
+
+    Reports images/s of both (and of the plain quotient instead of the
+    iterative compensation), the judge's FSC, and the FSC of the combined
+    spectrum to the plain iterative one up to 0.9 Nyquist."""
+    from rexlib_amd import learned, precision, synthetic
+    n = big = 256
+    rng = np.random.default_rng(seed)
+    kappa = float(rng.uniform(2.0, 20.0))
+    k = int(0.8 * images)
+    rot = np.concatenate([synthetic.von_mises_fisher_rotations(k, kappa, rng),
+                          synthetic.uniform_rotations(images - k, rng)])
+    rot = rot[rng.permutation(images)]
+    t0 = time.time()
+    vol = synthetic.blob_volume(big, 400, big * 0.375, (2.0, 8.0), seed=seed)
+    stack = synthetic.particle_stack(vol, images, n, seed=seed + 1, snr=0.1, rotations=rot, device="cuda")
+    log(f"learned form: generated {images} images of {n}^2 (kappa {kappa:.2f}) in {time.time() - t0:.1f} s")
+    transforms = torch.from_numpy(stack.transforms.astype(np.float32)).cuda()
+    ctf = torch.from_numpy(stack.ctf.astype(np.float32)).cuda()
+    radial = torch.ones(n // 2 + 1, device="cuda")
+    window = em.GriddingWindow.linear()
+    optics = em.CtfOptics(300.0, 2.7, 0.1)
+    launches = [(s, min(s + launch, images)) for s in range(0, images, launch)]
+    num = torch.zeros((big, big, big // 2 + 1), dtype=torch.complex64, device="cuda")
+    den = torch.zeros((big, big, big // 2 + 1), dtype=torch.float32, device="cuda")
+
+    def insert(staged, a, b):
+        a.zero_()
+        b.zero_()
+        for s, e in launches:
+            em.insert_ctf_images(a, b, staged[s:e], transforms[s:e], ctf[s:e], radial, window, optics, 1.5)
+
+    def plain():
+        insert(stack.images, num, den)
+        w = learned.iterative_weights(den)
+        return learned.reconstruct(num, w, window)
+
+    def plain_quotient():
+        insert(stack.images, num, den)
+        volume = rexlib_amd.irfft3(em.normalize(num, den, 1e-3))
+        return em.gridding_correct(volume, window, out=volume)
+
+    plain_ms = timed(torch, plain)
+    quotient_ms = timed(torch, plain_quotient)
+    insert(stack.images, num, den)
+    reference = num * learned.iterative_weights(den)
+    staged, _ = precision.stage_images(stack.images, "bf16")
+    del stack
+    torch.cuda.empty_cache()
+    num_c = torch.zeros_like(num)
+    den_c = torch.zeros_like(den)
+    insert(staged, num_c, den_c)  # eager pass: sizes every workspace before capture
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    capture = torch.cuda.Stream()
+    with torch.cuda.graph(graph, stream=capture):
+        insert(staged, num_c, den_c)
+    torch.cuda.synchronize()
+    model = learned.shipped_model()
+    t1 = time.perf_counter()
+    compensator = learned.Compensator(model, transforms, big)
+    graph.replay()
+    weights, judged = compensator(num_c, den_c)
+    torch.cuda.synchronize()
+    judged_ms = 1e3 * (time.perf_counter() - t1)
+
+    def combined():
+        graph.replay()
+        w, _ = compensator(num_c, den_c)
+        return learned.reconstruct(num_c, w, window)
+
+    combined_ms = timed(torch, combined)
+    graph.replay()
+    weights, steady = compensator(num_c, den_c)
+    agreement = precision.shell_report((num_c * weights).contiguous(), reference.contiguous(), 0.9)
+    out = {"images": images, "n": n, "volume": f"{big}^3", "kappa": kappa, "preferred_fraction": 0.8,
+           "launch": launch, "plain_fp32_iterative_ms": plain_ms, "plain_fp32_quotient_ms": quotient_ms,
+           "combined_ms": combined_ms, "combined_first_call_with_judge_ms": judged_ms,
+           "plain_images_per_s": images / (plain_ms * 1e-3),
+           "plain_quotient_images_per_s": images / (quotient_ms * 1e-3),
+           "combined_images_per_s": images / (combined_ms * 1e-3),
+           "speedup_vs_plain_iterative": plain_ms / combined_ms,
+           "speedup_vs_plain_quotient": quotient_ms / combined_ms,
+           "weights_used": steady["used"], "judge": {k: v for k, v in judged.items()},
+           "min_fsc_combined_to_plain_iterative_to_0.9_nyquist": agreement["min_fsc_to_cutoff"],
+           "max_shell_error_combined_to_plain_iterative": agreement["max_error_to_cutoff"],
+           "band_error_combined_to_plain_iterative": agreement["relative_error_to_cutoff"],
+           "model": "shipped" if model is not None else "absent (iterative weights)"}
+    log(f"learned form at its config: {out}")
+    del graph, staged, num, den, num_c, den_c, reference, weights, compensator
+    torch.cuda.empty_cache()
+    return out
+
+
+def precision_config(torch, em, rexlib_amd, images=200000, launch=4096, sample=256, seed=3000):
+    """The precision form at its own shape: `images` projections of 128^2
+    from a 128^3 blob volume (defocus 0.5-4 um, SNR 0.1), staged in HBM as
+    float32, bfloat16 and per-image scaled float16, padded on chip to 256^2
+    and inserted into 256^3 in launches of `launch`, then quotient, irfft3,
+    gridding correction and the crop of the central 128^3. The per-shell FSC
+    and relative error are taken on the first `sample` images against the
+    float64 oracle (pocketfft rfft2 of the padded float64 images, double
+    insertion), up to 0.9 Nyquist."""
+    import oracle
+    import scipy.fft
+    from rexlib_amd import precision, synthetic
+    m, n = 128, 256
+    t0 = time.time()
+    vol = synthetic.blob_volume(m, 120, 48.0, (1.5, 5.0), seed=seed)
+    stack = synthetic.particle_stack(vol, images, m, seed=seed + 1, snr=0.1, pixel_size=1.5,
+                                     defocus=(5000.0, 40000.0), device="cuda")
+    transforms = torch.from_numpy(stack.transforms.astype(np.float32)).cuda()
+    ctf = torch.from_numpy(stack.ctf.astype(np.float32)).cuda()
+    log(f"precision form: generated {images} images of {m}^2 in {time.time() - t0:.1f} s")
+    radial = torch.ones(n // 2 + 1, device="cuda")
+    window = em.GriddingWindow.linear()
+    optics = em.CtfOptics(300.0, 2.7, 0.1)
+    # The float64 oracle on the sample.
+    host = stack.images[:sample].cpu().numpy().astype(np.float64)
+    padded = np.zeros((sample, n, n))
+    o = n // 2 - m // 2
+    padded[:, o:o + m, o:o + m] = host
+    acc = np.zeros((n, n, n // 2 + 1), np.complex128)
+    ref = oracle.insert_f64("ctf", [acc, acc.real.copy(), scipy.fft.rfft2(padded, workers=min(16, os.cpu_count() or 1)),
+                                    stack.transforms[:sample], stack.ctf[:sample], np.ones(n // 2 + 1)],
+                            oracle.Window.linear(), pixel_size=1.5)
+    ref_q = torch.from_numpy(oracle.normalize(ref[0], ref[1]).astype(np.complex64)).cuda()
+    out = {"images": images, "image_extent": m, "padded_extent": n, "volume": f"{n}^3", "sample_for_fsc": sample}
+    for dtype in ("f32", "bf16", "fp16"):
+        staged, scales = precision.stage_images(stack.images, dtype)
+        num = torch.zeros((n, n, n // 2 + 1), dtype=torch.complex64, device="cuda")
+        den = torch.zeros((n, n, n // 2 + 1), dtype=torch.float32, device="cuda")
+
+        def insert(count):
+            num.zero_()
+            den.zero_()
+            for s in range(0, count, launch):
+                e = min(s + launch, count)
+                em.insert_ctf_images(num, den, staged[s:e], transforms[s:e], ctf[s:e], radial, window, optics, 1.5,
+                                     image_scales=None if scales is None else scales[s:e], padded=n)
+
+        def run():
+            insert(images)
+            volume = rexlib_amd.irfft3(em.normalize(num, den, 1e-3))
+            return em.gridding_correct(volume, window, crop=m)
+
+        ms = timed(torch, run)
+        insert(sample)
+        report = precision.shell_report(em.normalize(num, den, 1e-3), ref_q, 0.9)
+        out[dtype] = {"images_per_s": images / (ms * 1e-3), "ms": ms,
+                      "min_fsc_to_fp64_to_0.9_nyquist": report["min_fsc_to_cutoff"],
+                      "max_shell_error_to_0.9_nyquist": report["max_error_to_cutoff"],
+                      "staged_bytes_per_image": m * m * staged.element_size() + (4 if scales is not None else 0)}
+        del staged, scales, num, den
+    for dtype in ("bf16", "fp16"):
+        out[dtype]["vs_f32"] = out[dtype]["images_per_s"] / out["f32"]["images_per_s"]
+    log(f"precision form at its config: {out}")
+    del stack, transforms, ctf
+    torch.cuda.empty_cache()
+    return out
+
+
+def batched_config(torch, count=64, seed=5000, low=500, high=40000):
+    """The batched form at its own shape: `count` reconstructions, box sizes
+    drawn from {64, 96, 128, 192, 256, 320}, particle counts log-uniform in
+    [low, high], each with its own blob volume, CTF and SNR (0.05-0.5), float32
+    images; one HIP graph over four streams (autotuned brick per box size)
+    against the same reconstructions run one after another."""
     from rexlib_amd import batched, synthetic
     rng = np.random.default_rng(seed)
+    t0 = time.time()
     recons = []
     images = 0
+    sizes = []
     for i in range(count):
-        n = int(rng.choice([64, 96, 128]))
-        particles = int(np.exp(rng.uniform(np.log(500), np.log(4000))))
-        vol = synthetic.blob_volume(n, 30, n * 0.35, (1.0, 4.0), seed=seed + 10 * i)
+        n = int(rng.choice([64, 96, 128, 192, 256, 320]))
+        particles = int(np.exp(rng.uniform(np.log(low), np.log(high))))
+        vol = synthetic.blob_volume(n, 40, n * 0.35, (1.0, 0.02 * n + 1.0), seed=seed + 10 * i)
         stack = synthetic.particle_stack(vol, particles, n, seed=seed + 10 * i + 1, snr=float(rng.uniform(0.05, 0.5)),
                                          device="cuda")
         recons.append(batched.Reconstruction(stack.images, torch.from_numpy(stack.transforms.astype(np.float32)).cuda(),
                                              torch.from_numpy(stack.ctf.astype(np.float32)).cuda()))
         images += particles
+        sizes.append(n)
+    log(f"batched form: generated {count} reconstructions, {images} images, in {time.time() - t0:.1f} s")
     sequential_ms = timed(torch, lambda: batched.run_sequential(recons))
     graph = batched.BatchedReconstruction(recons, streams=4).capture()
-    graph_ms = timed(torch, graph.replay, repeats=3)
-    out = {"reconstructions": count, "images": images, "sequential_ms": sequential_ms, "graph_ms": graph_ms,
-           "speedup": sequential_ms / graph_ms, "graph_images_per_s": images / (graph_ms * 1e-3)}
-    log(f"batched form: {out}")
+    graph_ms = timed(torch, graph.replay, repeats=2)
+    out = {"reconstructions": count, "images": images, "box_sizes": {str(s): sizes.count(s) for s in sorted(set(sizes))},
+           "sequential_ms": sequential_ms, "graph_ms": graph_ms, "speedup": sequential_ms / graph_ms,
+           "sequential_images_per_s": images / (sequential_ms * 1e-3),
+           "graph_images_per_s": images / (graph_ms * 1e-3)}
+    log(f"batched form at its config: {out}")
+    del recons, graph
+    torch.cuda.empty_cache()
     return out
 
 
@@ -304,6 +491,9 @@ def main():
     parser.add_argument("--no-parity", action="store_true")
     parser.add_argument("--no-forms", action="store_true",
                         help="skip the precision, learned and batched legs reported beside the headline value")
+    parser.add_argument("--no-config-forms", action="store_true",
+                        help="skip the precision (200,000 128^2 images), batched (64 reconstructions) and learned-weight "
+                             "(80,000 256^2 images) legs")
     parser.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                         help="before the workload, copy a float32 buffer of MIB MiB three times with an elementwise "
                              "kernel (known traffic, to scale hardware byte counters)")
@@ -417,10 +607,14 @@ def main():
         torch.cuda.synchronize()
         reference = num.clone()
         forms = {
-            "precision": precision_forms(args, stack, transforms, ctf, radial, reference, torch, em, rexlib_amd),
-            "learned": learned_form(den, torch),
-            "batched": batched_form(torch),
+            "precision_on_this_workload": precision_forms(args, stack, transforms, ctf, radial, reference, torch, em,
+                                                          rexlib_amd),
+            "learned": learned_form(den, transforms, big, torch),
         }
+        if not args.no_config_forms:
+            forms["precision"] = precision_config(torch, em, rexlib_amd)
+            forms["batched"] = batched_config(torch)
+            forms["learned_weights"] = learned_config(torch, em, rexlib_amd)
         del reference
 
     parity = None if args.no_parity else parity_sample(args, stack, torch, em, rexlib_amd)

@@ -402,9 +402,25 @@ int rexlib_hip_shell_sums(const rexlib_hip_array *a, const rexlib_hip_array *b, 
 int rexlib_hip_density_compensation(const rexlib_hip_array *denominator, int32_t passes, double floor_value,
 	const rexlib_hip_array *weights, void *stream);
 
-/* features (N, N, N/2+1, 16) float16: per voxel |g|/(N/2), log(1+den), and
- * the 8 octant means of den around it (log(1+.)), then 6 zeros. */
-int rexlib_hip_compensation_features(const rexlib_hip_array *denominator, const rexlib_hip_array *features,
+/* table (8, 33, 64, 8) float32: the angular coverage of the slice normals
+ * of `transforms` (..., 3, 4) for a volume of extent big. Entry
+ * (r, t, p) is the radius (r + 0.5) / 8 * N/2 and the direction
+ * u = (sin T cos P, sin T sin P, cos T), T = t pi / 32 (poles and equator
+ * included), P = p 2 pi / 64 (x, y, z order, as the volume's frequencies);
+ * its 8 values are the fractions of the slices whose plane passes within one
+ * voxel of that point, widened by half the grid spacing
+ * (|n . u| < 1 / radius + 0.05), falling in each eighth of
+ * [0, pi) of the normal's angle around u, measured from d u / d T towards
+ * d u / d P; zeros when no plane passes. No reference counterpart. */
+int rexlib_hip_coverage_table(const rexlib_hip_array *transforms, int32_t big, const rexlib_hip_array *table,
+	void *stream);
+
+/* features (N, N, N/2+1, 16) float16: per voxel |g|/(N/2),
+ * log(1 + den * density_scale), and the 8 coverage fractions of the table
+ * entry of the voxel's radius bin floor(8 |g| / (N/2)) (at most 7) and of the
+ * grid direction nearest in T and in P (P wraps around), then 6 zeros. */
+int rexlib_hip_compensation_features(const rexlib_hip_array *denominator, const rexlib_hip_array *table,
+	double density_scale, const rexlib_hip_array *features,
 	void *stream);
 
 /* The perceptron 10 -> 64 -> 64 -> 64 -> 1 (ReLU) on v_mfma_f32_16x16x16_f16:

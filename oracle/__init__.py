@@ -436,31 +436,81 @@ def convolve_k3(field):
     return c
 
 
-def compensation_features(den):
-    """(N, N, N/2+1, 10): |g|/(N/2), log1p(den), and log1p of the mean of den
-    over the 7 neighbours of each octant, octants ordered (z, y, x) from
-    (-,-,-) with x fastest."""
+COVERAGE_SHAPE = (8, 33, 64, 8)
+
+
+def coverage_directions():
+    """Unit directions u and the frame (e1 = du/dtheta, e2 = du/dphi) of the
+    (theta, phi) grid of the coverage table, each (33, 64, 3): theta = k pi / 32
+    from pole to pole, phi = k 2 pi / 64."""
+    _, nt, npp, _ = COVERAGE_SHAPE
+    theta = np.arange(nt) * (math.pi / (nt - 1))
+    phi = np.arange(npp) * (2 * math.pi / npp)
+    t, p = np.meshgrid(theta, phi, indexing="ij")
+    u = np.stack([np.sin(t) * np.cos(p), np.sin(t) * np.sin(p), np.cos(t)], -1)
+    e1 = np.stack([np.cos(t) * np.cos(p), np.cos(t) * np.sin(p), -np.sin(t)], -1)
+    e2 = np.stack([-np.sin(p), np.cos(p), np.zeros_like(p)], -1)
+    return u, e1, e2
+
+
+def slice_normals(transforms):
+    """Unit normals X x Y of the slice planes (rows 0 and 1 of each transform)."""
+    t = np.asarray(transforms, np.float64).reshape(-1, 3, 4)
+    n = np.cross(t[:, 0, :3], t[:, 1, :3])
+    length = np.linalg.norm(n, axis=1, keepdims=True)
+    return np.where(length > 0, n / np.where(length > 0, length, 1), 0.0)
+
+
+def coverage_table(transforms, big):
+    """(8, 33, 64, 8): for radius (r + 0.5) / 8 * big / 2 and each direction
+    of the grid, the fractions of the slices with |n . u| < min(1, 1 / radius + 0.05)
+    in each eighth of [0, pi) of the normal's angle around u (zeros where no
+    plane passes)."""
+    nr, nt, npp, nb = COVERAGE_SHAPE
+    n = slice_normals(transforms)
+    u, e1, e2 = coverage_directions()
+    along = np.einsum("tpa,sa->tps", u, n)
+    a1 = np.einsum("tpa,sa->tps", e1, n)
+    a2 = np.einsum("tpa,sa->tps", e2, n)
+    angle = np.arctan2(a2, a1)
+    angle = np.where(angle < 0, angle + math.pi, angle)
+    angle = np.where(angle >= math.pi, angle - math.pi, angle)
+    bins = np.minimum(nb - 1, (angle * (nb / math.pi)).astype(np.int64))
+    out = np.zeros(COVERAGE_SHAPE)
+    for r in range(nr):
+        radius = (r + 0.5) / nr * (0.5 * big)
+        inside = (np.abs(along) < min(1.0, 1.0 / radius + 0.05)) & ((a1 != 0) | (a2 != 0))
+        for b in range(nb):
+            out[r, :, :, b] = (inside & (bins == b)).sum(-1)
+        total = out[r].sum(-1, keepdims=True)
+        out[r] = np.where(total > 0, out[r] / np.where(total > 0, total, 1), 0.0)
+    return out
+
+
+def compensation_features(den, table, scale):
+    """(N, N, N/2+1, 10): |g|/(N/2), log1p(den * scale), and the 8 coverage
+    fractions of the table entry nearest to the voxel's radius and direction
+    (radius bin floor(8 |g| / (N/2)), the grid direction nearest in theta and
+    in phi, phi wrapping around)."""
     extent = den.shape[0]
     h = extent // 2 + 1
-    d = np.maximum(full_from_half(np.asarray(den, dtype=np.float64), extent), 0.0)
+    nr, nt, npp, nb = COVERAGE_SHAPE
     f = np.fft.fftfreq(extent) * extent
-    fx = np.arange(h)
-    r = np.sqrt(f[:, None, None] ** 2 + f[None, :, None] ** 2 + fx[None, None, :] ** 2) / (extent / 2)
+    z = f[:, None, None] * np.ones((1, extent, h))
+    y = f[None, :, None] * np.ones((extent, 1, h))
+    x = np.arange(h)[None, None, :] * np.ones((extent, extent, 1))
+    length = np.sqrt(x * x + y * y + z * z)
+    rho = length / (extent / 2)
     out = np.zeros((extent, extent, h, 10))
-    out[..., 0] = r
-    out[..., 1] = np.log1p(d[:, :, :h])
-    b = 0
-    for oz in (-1, 1):
-        for oy in (-1, 1):
-            for ox in (-1, 1):
-                s = np.zeros_like(d)
-                for c in range(1, 8):
-                    dx = ox if c & 1 else 0
-                    dy = oy if c & 2 else 0
-                    dz = oz if c & 4 else 0
-                    s += np.roll(d, (-dz, -dy, -dx), axis=(0, 1, 2))
-                out[..., 2 + b] = np.log1p(s[:, :, :h] / 7.0)
-                b += 1
+    out[..., 0] = rho
+    out[..., 1] = np.log1p(np.maximum(np.asarray(den, np.float64), 0.0) * scale)
+    r = np.minimum(nr - 1, (rho * nr).astype(np.int64))
+    theta = np.where(length > 0, np.arccos(np.clip(z / np.where(length > 0, length, 1), -1, 1)), 0.0)
+    phi = np.arctan2(y, x)
+    phi = np.where(phi < 0, phi + 2 * math.pi, phi)
+    it = np.minimum(nt - 1, (theta * ((nt - 1) / math.pi) + 0.5).astype(np.int64))
+    ip = (phi * (npp / (2 * math.pi)) + 0.5).astype(np.int64) % npp
+    out[..., 2:] = np.asarray(table)[r, it, ip]
     return out
 
 

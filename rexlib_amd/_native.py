@@ -80,6 +80,7 @@ EXPORTS = [
     "rexlib_hip_where",
     "rexlib_hip_standardize_background",
     "rexlib_hip_density_compensation",
+    "rexlib_hip_coverage_table",
     "rexlib_hip_compensation_features",
     "rexlib_hip_mlp_parameter_count",
     "rexlib_hip_mlp_forward",
@@ -139,7 +140,8 @@ def library() -> ctypes.CDLL:
     d = ctypes.c_double
     lib.rexlib_hip_standardize_background.argtypes = [A, A, ctypes.c_uint32, d, d, d, A, vp]
     lib.rexlib_hip_density_compensation.argtypes = [A, ctypes.c_int32, d, A, vp]
-    lib.rexlib_hip_compensation_features.argtypes = [A, A, vp]
+    lib.rexlib_hip_coverage_table.argtypes = [A, ctypes.c_int32, A, vp]
+    lib.rexlib_hip_compensation_features.argtypes = [A, A, d, A, vp]
     lib.rexlib_hip_mlp_parameter_count.argtypes = []
     lib.rexlib_hip_mlp_forward.argtypes = [A, A, A, A, vp]
     lib.rexlib_hip_mlp_backward.argtypes = [A, A, A, A, A, A, d, vp]

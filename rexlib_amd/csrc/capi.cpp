@@ -950,24 +950,60 @@ int rexlib_hip_density_compensation(const rexlib_hip_array *denominator, int32_t
 	});
 }
 
-int rexlib_hip_compensation_features(const rexlib_hip_array *denominator, const rexlib_hip_array *features,
+int rexlib_hip_coverage_table(const rexlib_hip_array *transforms, int32_t big, const rexlib_hip_array *table,
 	void *stream)
 {
 	return guarded([&]
 	{
+		check_array(transforms, "coverage_table transforms", 2);
+		check_array(table, "coverage_table output", 4);
+		expect_dtype(*transforms, REXLIB_HIP_FLOAT32, "coverage_table transforms");
+		expect_dtype(*table, REXLIB_HIP_FLOAT32, "coverage_table output");
+		expect_core(*transforms, "coverage_table transforms", {3, 4});
+		expect_core(*table, "coverage_table output", {8, 33, 64, 8});
+		if (table->rank != 4 || !contiguous(*table))
+		{
+			fail(REXLIB_HIP_UNSUPPORTED, "coverage_table: a contiguous (8, 33, 64, 8) table");
+		}
+		if (big < 2)
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "coverage_table: the volume extent must be at least 2");
+		}
+		int64_t slices = 0, stride = 0;
+		if (!collapse_batch(*transforms, 2, slices, stride))
+		{
+			fail(REXLIB_HIP_UNSUPPORTED, "coverage_table: batch axes must collapse to one strided axis");
+		}
+		launch_coverage(static_cast<const float *>(transforms->data), slices, stride, core_stride(*transforms, 2),
+			core_stride(*transforms, 1), big, static_cast<float *>(table->data), as_stream(stream));
+	});
+}
+
+int rexlib_hip_compensation_features(const rexlib_hip_array *denominator, const rexlib_hip_array *table,
+	double density_scale, const rexlib_hip_array *features, void *stream)
+{
+	return guarded([&]
+	{
 		check_array(denominator, "compensation_features denominator", 3);
+		check_array(table, "compensation_features table", 4);
 		check_array(features, "compensation_features output", 4);
 		expect_dtype(*denominator, REXLIB_HIP_FLOAT32, "compensation_features denominator");
+		expect_dtype(*table, REXLIB_HIP_FLOAT32, "compensation_features table");
 		expect_dtype(*features, REXLIB_HIP_FLOAT16, "compensation_features output");
 		const int64_t big = core_extent(*denominator, 3);
 		expect_core(*denominator, "compensation_features denominator", {big, big, big / 2 + 1});
+		expect_core(*table, "compensation_features table", {8, 33, 64, 8});
 		expect_core(*features, "compensation_features output", {big, big, big / 2 + 1, 16});
-		if (denominator->rank != 3 || !contiguous(*denominator) || !contiguous(*features))
+		if (denominator->rank != 3 || !contiguous(*denominator) || !contiguous(*features) || !contiguous(*table))
 		{
 			fail(REXLIB_HIP_UNSUPPORTED, "compensation_features: contiguous operands of one grid");
 		}
-		launch_features(static_cast<const float *>(denominator->data), features->data, static_cast<int>(big),
-			as_stream(stream));
+		if (!(density_scale > 0.0) || !std::isfinite(density_scale))
+		{
+			fail(REXLIB_HIP_INVALID_ARGUMENT, "compensation_features: the density scale must be finite and positive");
+		}
+		launch_features(static_cast<const float *>(denominator->data), static_cast<const float *>(table->data),
+			static_cast<float>(density_scale), features->data, static_cast<int>(big), as_stream(stream));
 	});
 }
 

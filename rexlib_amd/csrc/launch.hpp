@@ -118,7 +118,12 @@ void launch_gridding(const float *in, float *out, int big, int crop, int64_t vol
 int mlp_parameter_count();
 void launch_compensation(const float *den, float *w, float *scratch, int big, int passes, float floor_value,
 	hipStream_t stream);
-void launch_features(const float *den, void *features, int big, hipStream_t stream);
+// Coverage table (8 radii x 32 theta x 64 phi x 8 angle bins, float32) of
+// the slice normals of `slices` transforms (strides in elements).
+void launch_coverage(const float *transforms, int64_t slices, int64_t ts0, int64_t ts1, int64_t ts2, int big,
+	float *table, hipStream_t stream);
+void launch_features(const float *den, const float *table, float density_scale, void *features, int big,
+	hipStream_t stream);
 void launch_mlp_forward(const void *features, const void *params_h, const float *params_f, float *out, int64_t count,
 	hipStream_t stream);
 void launch_mlp_backward(const void *features, const float *target, const void *params_h, const float *params_f,

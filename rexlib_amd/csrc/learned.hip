@@ -102,9 +102,115 @@ __global__ void fill_kernel(float *out, int64_t count, float value)
 	}
 }
 
-// Features of every voxel, as 16 float16 per voxel (the last 6 zero); the
-// 26 neighbours are read once and the 32 bytes stored as two 16-byte words.
-__global__ __launch_bounds__(256) void features_kernel(const float *den, __half *features, int big)
+// Angular coverage of the slice normals: for radius bin r (of the 8
+// spanning |g| in [0, N/2]) and direction u on a 33 x 64 (theta, phi) grid
+// (theta = k pi / 32 from pole to pole, phi = k 2 pi / 64: the axes and the
+// coordinate planes of the lattice fall on grid directions, so a voxel's
+// nearest direction is never a tie between two),
+// the fractions of the slices whose plane passes within one voxel of the
+// point at that radius and direction, widened by half the grid's spacing
+// (|n . u| < 1 / radius + 0.05, so that no plane slips between directions), that fall in
+// each of 8 bins of the normal's angle around u, taken modulo pi (n and -n
+// are one plane), measured from the local frame e1 = d u / d theta,
+// e2 = d u / d phi. All zeros where no plane passes.
+constexpr int coverage_radii = 8;
+constexpr int coverage_theta = 33;
+constexpr int coverage_phi = 64;
+constexpr int coverage_bins = 8;
+constexpr int coverage_tile = 256;
+
+__device__ inline float3 coverage_direction(int it, int ip, float3 &e1, float3 &e2)
+{
+	const float theta = static_cast<float>(it) * (3.14159265358979f / (coverage_theta - 1));
+	const float phi = static_cast<float>(ip) * (6.28318530717959f / coverage_phi);
+	float st, ct, sp, cp;
+	sincosf(theta, &st, &ct);
+	sincosf(phi, &sp, &cp);
+	e1 = make_float3(ct * cp, ct * sp, -st);
+	e2 = make_float3(-sp, cp, 0.0f);
+	return make_float3(st * cp, st * sp, ct);
+}
+
+// One thread per table entry; the workgroup stages the unit normals of 256
+// slices at a time in LDS.
+__global__ __launch_bounds__(256) void coverage_kernel(const float *transforms, int64_t slices, int64_t ts0,
+	int64_t ts1, int64_t ts2, int big, float *table)
+{
+	__shared__ float3 normals[coverage_tile];
+	const int entry = blockIdx.x * blockDim.x + threadIdx.x;
+	constexpr int entries = coverage_radii * coverage_theta * coverage_phi;
+	const bool live = entry < entries;
+	const int r = entry / (coverage_theta * coverage_phi);
+	const int rest = entry - r * coverage_theta * coverage_phi;
+	const int it = rest / coverage_phi;
+	const int ip = rest - it * coverage_phi;
+	float3 e1, e2;
+	const float3 u = coverage_direction(it, ip, e1, e2);
+	const float radius = (static_cast<float>(r) + 0.5f) / coverage_radii * (0.5f * static_cast<float>(big));
+	const float band = fminf(1.0f, 1.0f / radius + 0.05f);
+	float count[coverage_bins];
+	for (int b = 0; b < coverage_bins; ++b)
+	{
+		count[b] = 0.0f;
+	}
+	for (int64_t first = 0; first < slices; first += coverage_tile)
+	{
+		const int64_t j = first + threadIdx.x;
+		float3 n = make_float3(0.0f, 0.0f, 0.0f);
+		if (j < slices)
+		{
+			const float *t = transforms + j * ts0;
+			const float x0 = t[0], x1 = t[ts2], x2 = t[2 * ts2];
+			const float y0 = t[ts1], y1 = t[ts1 + ts2], y2 = t[ts1 + 2 * ts2];
+			n = make_float3(x1 * y2 - x2 * y1, x2 * y0 - x0 * y2, x0 * y1 - x1 * y0);
+			const float length = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+			const float inv = length > 0.0f ? 1.0f / length : 0.0f;
+			n = make_float3(n.x * inv, n.y * inv, n.z * inv);
+		}
+		__syncthreads();
+		normals[threadIdx.x] = n;
+		__syncthreads();
+		const int tile = static_cast<int>(min<int64_t>(coverage_tile, slices - first));
+		for (int k = 0; k < tile; ++k)
+		{
+			const float3 m = normals[k];
+			const float along = m.x * u.x + m.y * u.y + m.z * u.z;
+			const float a1 = m.x * e1.x + m.y * e1.y + m.z * e1.z;
+			const float a2 = m.x * e2.x + m.y * e2.y + m.z * e2.z;
+			const bool inside = fabsf(along) < band && (a1 != 0.0f || a2 != 0.0f);
+			// The angle modulo pi, in [0, pi).
+			float angle = atan2f(a2, a1);
+			angle = angle < 0.0f ? angle + 3.14159265358979f : angle;
+			angle = angle >= 3.14159265358979f ? angle - 3.14159265358979f : angle;
+			const int bin = min(coverage_bins - 1, static_cast<int>(angle * (coverage_bins / 3.14159265358979f)));
+			for (int b = 0; b < coverage_bins; ++b)
+			{
+				count[b] += inside && bin == b ? 1.0f : 0.0f;
+			}
+		}
+	}
+	if (live)
+	{
+		float total = 0.0f;
+		for (int b = 0; b < coverage_bins; ++b)
+		{
+			total += count[b];
+		}
+		const float scale = total > 0.0f ? 1.0f / total : 0.0f;
+		for (int b = 0; b < coverage_bins; ++b)
+		{
+			table[static_cast<int64_t>(entry) * coverage_bins + b] = count[b] * scale;
+		}
+	}
+}
+
+// Features of every voxel, as 16 float16 per voxel (the last 6 zero):
+// the shell radius |g| / (N/2), log(1 + den * density_scale) (the local
+// sample count against the mean of the positive weights), and the 8
+// coverage fractions of the table entry nearest to the voxel's radius and
+// direction. The 32 bytes are stored as two 16-byte words.
+__global__ __launch_bounds__(256) void features_kernel(const float *den, const float *table, float density_scale,
+	__half *features, int big)
 {
 	const int half = big / 2 + 1;
 	int x, sy, sz;
@@ -113,20 +219,9 @@ __global__ __launch_bounds__(256) void features_kernel(const float *den, __half 
 		return;
 	}
 	const int64_t i = (static_cast<int64_t>(sz) * big + sy) * half + x;
-	const int y = sy <= big / 2 ? sy : sy - big;
-	const int z = sz <= big / 2 ? sz : sz - big;
-	float nb[27];
-	for (int dz = -1; dz <= 1; ++dz)
-	{
-		for (int dy = -1; dy <= 1; ++dy)
-		{
-			for (int dx = -1; dx <= 1; ++dx)
-			{
-				nb[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1] =
-					fmaxf(den[neighbour_index(x + dx, y + dy, z + dz, big, half)], 0.0f);
-			}
-		}
-	}
+	// Signed frequencies as numpy's fftfreq: index N/2 is -N/2.
+	const int y = sy < big / 2 ? sy : sy - big;
+	const int z = sz < big / 2 ? sz : sz - big;
 	// log(1 + v) by the hardware logarithm, corrected for the rounding of
 	// 1 + v: with u = fl(1 + v), log(u) v / (u - 1) keeps the relative error
 	// near one ulp for small v as well (u = 1 gives v itself).
@@ -135,28 +230,21 @@ __global__ __launch_bounds__(256) void features_kernel(const float *den, __half 
 		const float u = 1.0f + v;
 		return u == 1.0f ? v : __logf(u) * (v / (u - 1.0f));
 	};
+	const float length = sqrtf(static_cast<float>(x * x + y * y + z * z));
 	float f[16];
-	f[0] = sqrtf(static_cast<float>(x * x + y * y + z * z)) / (0.5f * static_cast<float>(big));
-	f[1] = log1(fmaxf(den[i], 0.0f));
-	int bin = 0;
-	for (int oz = -1; oz <= 1; oz += 2)
+	f[0] = length / (0.5f * static_cast<float>(big));
+	f[1] = log1(fmaxf(den[i], 0.0f) * density_scale);
+	const int r = min(coverage_radii - 1, static_cast<int>(f[0] * coverage_radii));
+	const float theta = length > 0.0f ? acosf(fminf(1.0f, fmaxf(-1.0f, static_cast<float>(z) / length))) : 0.0f;
+	float phi = atan2f(static_cast<float>(y), static_cast<float>(x));
+	phi = phi < 0.0f ? phi + 6.28318530717959f : phi;
+	// The nearest grid direction (phi wraps around).
+	const int it = min(coverage_theta - 1, static_cast<int>(theta * ((coverage_theta - 1) / 3.14159265358979f) + 0.5f));
+	const int ip = static_cast<int>(phi * (coverage_phi / 6.28318530717959f) + 0.5f) % coverage_phi;
+	const float *entry = table + (static_cast<int64_t>(r * coverage_theta + it) * coverage_phi + ip) * coverage_bins;
+	for (int b = 0; b < coverage_bins; ++b)
 	{
-		for (int oy = -1; oy <= 1; oy += 2)
-		{
-			for (int ox = -1; ox <= 1; ox += 2)
-			{
-				float sum = 0.0f;
-				for (int c = 1; c < 8; ++c)
-				{
-					const int dx = (c & 1) ? ox : 0;
-					const int dy = (c & 2) ? oy : 0;
-					const int dz = (c & 4) ? oz : 0;
-					sum += nb[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1];
-				}
-				f[2 + bin] = log1(sum / 7.0f);
-				++bin;
-			}
-		}
+		f[2 + b] = entry[b];
 	}
 	for (int k = 10; k < 16; ++k)
 	{
@@ -715,9 +803,19 @@ void launch_compensation(const float *den, float *w, float *scratch, int big, in
 	REXLIB_HIP_CHECK(hipGetLastError());
 }
 
-void launch_features(const float *den, void *features, int big, hipStream_t stream)
+void launch_coverage(const float *transforms, int64_t slices, int64_t ts0, int64_t ts1, int64_t ts2, int big,
+	float *table, hipStream_t stream)
 {
-	features_kernel<<<volume_grid(big), 256, 0, stream>>>(den, static_cast<__half *>(features), big);
+	constexpr int entries = coverage_radii * coverage_theta * coverage_phi;
+	coverage_kernel<<<(entries + 255) / 256, 256, 0, stream>>>(transforms, slices, ts0, ts1, ts2, big, table);
+	REXLIB_HIP_CHECK(hipGetLastError());
+}
+
+void launch_features(const float *den, const float *table, float density_scale, void *features, int big,
+	hipStream_t stream)
+{
+	features_kernel<<<volume_grid(big), 256, 0, stream>>>(den, table, density_scale, static_cast<__half *>(features),
+		big);
 	REXLIB_HIP_CHECK(hipGetLastError());
 }
 

@@ -36,7 +36,8 @@ def shell_report(test: torch.Tensor, reference: torch.Tensor, cutoff: float = 0.
     Returns a dict with the shell index, the FSC, the relative error
     sqrt(sum |a-b|^2 / sum |b|^2) per shell, and their worst values over the
     shells up to `cutoff` of Nyquist (shell 0 excluded from the minimum FSC if
-    it holds a single element)."""
+    it holds a single element), and the relative error of the whole band of
+    shells 1 to that cutoff."""
     big = test.shape[-3]
     s = em.shell_sums(test, reference)
     cross, aa, bb, count = s[0], s[1], s[2], s[3]
@@ -44,6 +45,8 @@ def shell_report(test: torch.Tensor, reference: torch.Tensor, cutoff: float = 0.
     # |a - b|^2 = |a|^2 + |b|^2 - 2 Re(a conj b)
     err = torch.sqrt(torch.clamp(aa + bb - 2 * cross, min=0.0) / torch.clamp(bb, min=1e-300))
     limit = int(cutoff * (big // 2))
+    band = torch.clamp((aa + bb - 2 * cross)[1:limit + 1].sum(), min=0.0) / torch.clamp(bb[1:limit + 1].sum(),
+                                                                                          min=1e-300)
     fsc_h = fsc.cpu()
     err_h = err.cpu()
     return {
@@ -53,5 +56,6 @@ def shell_report(test: torch.Tensor, reference: torch.Tensor, cutoff: float = 0.
         "min_fsc_to_cutoff": float(fsc_h[1:limit + 1].min()),
         "max_error_to_cutoff": float(err_h[1:limit + 1].max()),
         "cutoff_shell": limit,
+        "relative_error_to_cutoff": float(torch.sqrt(band)),
         "count": count.cpu().tolist(),
     }

@@ -82,14 +82,52 @@ def test_full_from_half_is_hermitian_for_a_real_field():
     np.testing.assert_allclose(got, np.abs(spec_full), atol=1e-9)
 
 
-def test_features_at_the_origin():
+def test_features_at_the_origin_and_on_the_axes():
     den = np.zeros((8, 8, 5))
     den[0, 0, 0] = np.e - 1
-    f = oracle.compensation_features(den)
+    table = np.zeros(oracle.COVERAGE_SHAPE)
+    table[0, 0, 0] = np.arange(8)
+    table[3, 16, 0] = 10 + np.arange(8)  # theta = pi/2, phi = 0: the +x axis
+    f = oracle.compensation_features(den, table, 1.0)
     assert f[0, 0, 0, 0] == 0.0
     assert f[0, 0, 0, 1] == pytest.approx(1.0)
-    # The neighbour (1, 0, 0) sees the origin in the octants with ox = -1.
-    assert f[0, 0, 1, 2] == pytest.approx(np.log1p((np.e - 1) / 7))
+    assert np.array_equal(f[0, 0, 0, 2:], np.arange(8))
+    # g = (x, y, z) = (2, 0, 0): radius 2 / 4, bin floor(0.5 * 8) = 4? No:
+    # (4, 0, 0) is radius 1, bin 7; (1, 0, 0) is radius 1/4, bin 2.
+    assert f[0, 0, 1, 0] == 0.25
+    table2 = np.zeros(oracle.COVERAGE_SHAPE)
+    table2[2, 16, 0] = 1 + np.arange(8)
+    f2 = oracle.compensation_features(den, table2, 1.0)
+    assert np.array_equal(f2[0, 0, 1, 2:], 1 + np.arange(8))
+
+
+def test_coverage_of_one_plane():
+    # One slice in the xy plane (normal +z): the planes through the points of
+    # the equator (u perpendicular to z) hold it, at the angle of -e1 = +z
+    # around u, i.e. pi, which is 0 modulo pi: bin 0. Away from the equator
+    # by more than the band it is not counted.
+    t = np.zeros((1, 3, 4))
+    t[0, 0, 0] = 1.0
+    t[0, 1, 1] = 1.0
+    table = oracle.coverage_table(t, 64)
+    equator = table[:, 16]  # theta = pi/2
+    assert np.all(equator[..., 0] == 1.0) and np.all(equator[..., 1:] == 0.0)
+    assert np.all(table[7, 0] == 0.0)  # near the pole, far from the plane at radius ~30
+    assert np.all(table[7, 16].sum(-1) == 1.0)
+    # At radius 30 the band (0.083) is narrower than the grid step (pi / 32):
+    # the neighbouring latitudes do not hold the plane.
+    assert np.all(table[7, 15] == 0.0) and np.all(table[7, 17] == 0.0)
+
+
+def test_coverage_fractions_sum_to_one_where_planes_pass():
+    rng = np.random.default_rng(1)
+    from rexlib_amd import synthetic
+    t = synthetic.transforms_of(synthetic.uniform_rotations(500, rng))
+    table = oracle.coverage_table(t, 32)
+    totals = table.sum(-1)
+    assert np.allclose(totals[totals > 0], 1.0)
+    # Radius bin 0 (radius 1 voxel: band 1) holds every plane everywhere.
+    assert np.all(totals[0] > 0.999)
 
 
 def test_separable_k3_equals_the_27_point_sum():
@@ -102,3 +140,16 @@ def test_separable_k3_equals_the_27_point_sum():
     assert c[2, 2, 2] == pytest.approx(8 / 27)
     assert c[1, 1, 1] == pytest.approx(1 / 216)
     assert c.sum() == pytest.approx(1.0)
+
+
+def test_features_take_the_nyquist_index_as_negative():
+    # Index N/2 along y or z is the frequency -N/2 (numpy's fftfreq): the
+    # voxel (z, y, x) = (N/2, 0, 0) lies on the -z axis (theta = pi, bin 32),
+    # (0, N/2, 0) on the -y axis (phi = 3 pi / 2, bin 48).
+    den = np.ones((16, 16, 9))
+    table = np.zeros(oracle.COVERAGE_SHAPE)
+    table[7, 32, 0] = np.arange(8)
+    table[7, 16, 48] = 10 + np.arange(8)
+    f = oracle.compensation_features(den, table, 1.0)
+    assert np.array_equal(f[8, 0, 0, 2:], np.arange(8))
+    assert np.array_equal(f[0, 8, 0, 2:], 10 + np.arange(8))
