@@ -295,10 +295,12 @@ def learned_config(torch, em, rexlib_amd, images=80000, launch=4096, seed=7000):
     torch.cuda.empty_cache()
     num_c = torch.zeros_like(num)
     den_c = torch.zeros_like(den)
-    insert(staged, num_c, den_c)  # eager pass: sizes every workspace before capture
+    capture = torch.cuda.Stream()
+    capture.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(capture):
+        insert(staged, num_c, den_c)  # eager pass on the capture stream: sizes its workspace
     torch.cuda.synchronize()
     graph = torch.cuda.CUDAGraph()
-    capture = torch.cuda.Stream()
     with torch.cuda.graph(graph, stream=capture):
         insert(staged, num_c, den_c)
     torch.cuda.synchronize()

@@ -131,10 +131,13 @@ __device__ inline float3 coverage_direction(int it, int ip, float3 &e1, float3 &
 	return make_float3(st * cp, st * sp, ct);
 }
 
-// One thread per table entry; the workgroup stages the unit normals of 256
-// slices at a time in LDS.
-__global__ __launch_bounds__(256) void coverage_kernel(const float *transforms, int64_t slices, int64_t ts0,
-	int64_t ts1, int64_t ts2, int big, float *table)
+// One thread per table entry and one slice range per blockIdx.y; the
+// workgroup stages the unit normals of 256 slices at a time in LDS. The
+// counts of the range are added to `counts` (the table's own memory, as
+// uint32, zeroed first) with integer atomics, so the result does not depend
+// on the order of the ranges.
+__global__ __launch_bounds__(256) void coverage_count_kernel(const float *transforms, int64_t slices, int64_t ts0,
+	int64_t ts1, int64_t ts2, int big, int64_t per_block, unsigned *counts)
 {
 	__shared__ float3 normals[coverage_tile];
 	const int entry = blockIdx.x * blockDim.x + threadIdx.x;
@@ -148,16 +151,18 @@ __global__ __launch_bounds__(256) void coverage_kernel(const float *transforms, 
 	const float3 u = coverage_direction(it, ip, e1, e2);
 	const float radius = (static_cast<float>(r) + 0.5f) / coverage_radii * (0.5f * static_cast<float>(big));
 	const float band = fminf(1.0f, 1.0f / radius + 0.05f);
-	float count[coverage_bins];
+	unsigned count[coverage_bins];
 	for (int b = 0; b < coverage_bins; ++b)
 	{
-		count[b] = 0.0f;
+		count[b] = 0;
 	}
-	for (int64_t first = 0; first < slices; first += coverage_tile)
+	const int64_t begin = blockIdx.y * per_block;
+	const int64_t end = min<int64_t>(slices, begin + per_block);
+	for (int64_t first = begin; first < end; first += coverage_tile)
 	{
 		const int64_t j = first + threadIdx.x;
 		float3 n = make_float3(0.0f, 0.0f, 0.0f);
-		if (j < slices)
+		if (j < end)
 		{
 			const float *t = transforms + j * ts0;
 			const float x0 = t[0], x1 = t[ts2], x2 = t[2 * ts2];
@@ -170,37 +175,60 @@ __global__ __launch_bounds__(256) void coverage_kernel(const float *transforms, 
 		__syncthreads();
 		normals[threadIdx.x] = n;
 		__syncthreads();
-		const int tile = static_cast<int>(min<int64_t>(coverage_tile, slices - first));
+		const int tile = static_cast<int>(min<int64_t>(coverage_tile, end - first));
 		for (int k = 0; k < tile; ++k)
 		{
 			const float3 m = normals[k];
 			const float along = m.x * u.x + m.y * u.y + m.z * u.z;
 			const float a1 = m.x * e1.x + m.y * e1.y + m.z * e1.z;
 			const float a2 = m.x * e2.x + m.y * e2.y + m.z * e2.z;
-			const bool inside = fabsf(along) < band && (a1 != 0.0f || a2 != 0.0f);
-			// The angle modulo pi, in [0, pi).
-			float angle = atan2f(a2, a1);
-			angle = angle < 0.0f ? angle + 3.14159265358979f : angle;
-			angle = angle >= 3.14159265358979f ? angle - 3.14159265358979f : angle;
-			const int bin = min(coverage_bins - 1, static_cast<int>(angle * (coverage_bins / 3.14159265358979f)));
-			for (int b = 0; b < coverage_bins; ++b)
+			if (fabsf(along) < band && (a1 != 0.0f || a2 != 0.0f))
 			{
-				count[b] += inside && bin == b ? 1.0f : 0.0f;
+				// The angle modulo pi, in [0, pi).
+				float angle = atan2f(a2, a1);
+				angle = angle < 0.0f ? angle + 3.14159265358979f : angle;
+				angle = angle >= 3.14159265358979f ? angle - 3.14159265358979f : angle;
+				const int bin = min(coverage_bins - 1, static_cast<int>(angle * (coverage_bins / 3.14159265358979f)));
+				for (int b = 0; b < coverage_bins; ++b)
+				{
+					count[b] += bin == b ? 1u : 0u;
+				}
 			}
 		}
 	}
 	if (live)
 	{
-		float total = 0.0f;
 		for (int b = 0; b < coverage_bins; ++b)
 		{
-			total += count[b];
+			if (count[b] != 0)
+			{
+				atomicAdd(counts + static_cast<int64_t>(entry) * coverage_bins + b, count[b]);
+			}
 		}
-		const float scale = total > 0.0f ? 1.0f / total : 0.0f;
-		for (int b = 0; b < coverage_bins; ++b)
-		{
-			table[static_cast<int64_t>(entry) * coverage_bins + b] = count[b] * scale;
-		}
+	}
+}
+
+// In place: the uint32 counts of each entry become the fractions of their
+// total (zeros when no plane passes).
+__global__ __launch_bounds__(256) void coverage_normalize_kernel(float *table)
+{
+	const int entry = blockIdx.x * blockDim.x + threadIdx.x;
+	if (entry >= coverage_radii * coverage_theta * coverage_phi)
+	{
+		return;
+	}
+	const unsigned *counts = reinterpret_cast<const unsigned *>(table) + static_cast<int64_t>(entry) * coverage_bins;
+	unsigned c[coverage_bins];
+	unsigned total = 0;
+	for (int b = 0; b < coverage_bins; ++b)
+	{
+		c[b] = counts[b];
+		total += c[b];
+	}
+	const float scale = total > 0 ? 1.0f / static_cast<float>(total) : 0.0f;
+	for (int b = 0; b < coverage_bins; ++b)
+	{
+		table[static_cast<int64_t>(entry) * coverage_bins + b] = static_cast<float>(c[b]) * scale;
 	}
 }
 
@@ -807,7 +835,17 @@ void launch_coverage(const float *transforms, int64_t slices, int64_t ts0, int64
 	float *table, hipStream_t stream)
 {
 	constexpr int entries = coverage_radii * coverage_theta * coverage_phi;
-	coverage_kernel<<<(entries + 255) / 256, 256, 0, stream>>>(transforms, slices, ts0, ts1, ts2, big, table);
+	constexpr int64_t table_size = static_cast<int64_t>(entries) * coverage_bins;
+	// Up to 64 slice ranges of whole tiles: 66 x 64 workgroups for large stacks.
+	const int64_t tiles = std::max<int64_t>(1, (slices + coverage_tile - 1) / coverage_tile);
+	const int64_t per_block = ((tiles + 63) / 64) * coverage_tile;
+	const unsigned ranges = static_cast<unsigned>(std::max<int64_t>(1, (slices + per_block - 1) / per_block));
+	fill_kernel<<<(table_size + 255) / 256, 256, 0, stream>>>(table, table_size, 0.0f);
+	REXLIB_HIP_CHECK(hipGetLastError());
+	coverage_count_kernel<<<dim3((entries + 255) / 256, ranges), 256, 0, stream>>>(transforms, slices, ts0, ts1, ts2,
+		big, per_block, reinterpret_cast<unsigned *>(table));
+	REXLIB_HIP_CHECK(hipGetLastError());
+	coverage_normalize_kernel<<<(entries + 255) / 256, 256, 0, stream>>>(table);
 	REXLIB_HIP_CHECK(hipGetLastError());
 }
 
