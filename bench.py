@@ -232,6 +232,36 @@ def learned_form(den, transforms, big, torch):
     return out
 
 
+def mlp_profile(torch, big=256, rows=65536, seed=11):
+    """The perceptron alone: features of a random (big, big, big/2+1) weight
+    volume with a coverage table of 50,000 uniform orientations, the forward
+    pass over every voxel three times, and three training steps (forward,
+    backward, Adam) on `rows` of those features."""
+    from rexlib_amd import learned, synthetic
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    den = torch.rand((big, big, big // 2 + 1), device="cuda", generator=g) + 0.5
+    rot = synthetic.uniform_rotations(50000, np.random.default_rng(seed))
+    table = learned.coverage_table(torch.from_numpy(synthetic.transforms_of(rot).astype(np.float32)).cuda(), big)
+    f = learned.features(den, table).reshape(-1, learned.FEATURES)
+    model = learned.CompensationModel(seed=seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        out = model.predict(f)
+    torch.cuda.synchronize()
+    forward_ms = 1e3 * (time.perf_counter() - t0) / 3
+    target = torch.zeros(rows, device="cuda")
+    batch = f[:rows].contiguous()
+    for _ in range(3):
+        model.train_step(batch, target)
+    torch.cuda.synchronize()
+    flops = 2.0 * (10 * 64 + 64 * 64 + 64 * 64 + 64 * 1) * f.shape[0]
+    return {"mlp_profile": {"voxels": int(f.shape[0]), "forward_ms": forward_ms,
+                            "forward_tflops": flops / (forward_ms * 1e-3) / 1e12, "train_rows": rows,
+                            "finite": bool(torch.isfinite(out).all())}}
+
+
 def learned_config(torch, em, rexlib_amd, images=80000, launch=4096, seed=7000):
     """The learned-weight form at its own shape: 256^3 from `images`
     projections of 256^2 of a 400-blob volume (CTF defocus 1-3 um, SNR 0.1),
@@ -499,6 +529,10 @@ def main():
     parser.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                         help="before the workload, copy a float32 buffer of MIB MiB three times with an elementwise "
                              "kernel (known traffic, to scale hardware byte counters)")
+    parser.add_argument("--mlp-profile", action="store_true",
+                        help="only the perceptron: features of a random 256^3 weight volume, the forward pass over "
+                             "every voxel three times and three training steps on 65,536 rows (the workload of the "
+                             "MFMA profile), then exit")
     args = parser.parse_args()
 
     if args.gpus != 1:
@@ -512,6 +546,9 @@ def main():
     from rexlib_amd import _native, em, tune
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.mlp_profile:
+        print(json.dumps(mlp_profile(torch)), flush=True)
+        return
     if args.copy_calibration:
         src = torch.ones(args.copy_calibration * (1 << 18), device="cuda")
         dst = torch.empty_like(src)

@@ -891,15 +891,12 @@ int rexlib_hip_insert_ctf_images(const rexlib_hip_array *numerator, const rexlib
 		const int64_t pitch = (h + 15) / 16 * 16;
 		float2 *spectra = static_cast<float2 *>(workspace_area_bytes(area_spectra,
 			sizeof(float2) * static_cast<std::size_t>(std::max<int64_t>(batch, 1) * n * pitch), s));
-		if (batch > 0)
-		{
-			const line_set in_rows{batch * m, 1, m, in_stride, 0, core_stride(*images, 2), core_stride(*images, 1)};
-			const line_set out_rows{batch * n, 1, n, n * pitch, 0, pitch, 1};
-			launch_rfft_rows(kind, images->data, in_rows, static_cast<int>(m), scales, scale_stride, spectra, out_rows,
-				n, s);
-		}
-		// The column transforms are left to the insertion.
-		const pending_columns columns{spectra, line_set{batch * h, 1, h, n * pitch, 0, 1, pitch}};
+		// The row and column transforms are left to the insertion, which runs
+		// them chunk by chunk beside the insertion of the chunk before.
+		const pending_rows rows{kind, images->data, kind == real_f32 ? sizeof(float) : sizeof(uint16_t),
+			line_set{batch * m, 1, m, in_stride, 0, core_stride(*images, 2), core_stride(*images, 1)},
+			static_cast<int>(m), scales, scale_stride, line_set{batch * n, 1, n, n * pitch, 0, pitch, 1}, n};
+		const pending_columns columns{spectra, line_set{batch * h, 1, h, n * pitch, 0, 1, pitch}, &rows};
 		rexlib_hip_array slices{};
 		slices.data = spectra;
 		slices.dtype = REXLIB_HIP_COMPLEX64;

@@ -1543,6 +1543,25 @@ void clear_insertion_tuning()
 	tuning_table.clear();
 }
 
+// The row transforms of images [first, last) of a pending image insertion.
+void launch_rows(const pending_columns &columns, int64_t first, int64_t last, hipStream_t stream)
+{
+	const pending_rows &rows = *columns.rows;
+	if (last <= first)
+	{
+		return;
+	}
+	line_set in = rows.in_rows;
+	in.count = (last - first) * rows.m;
+	line_set out = rows.out_rows;
+	out.count = (last - first) * rows.n;
+	const char *images = static_cast<const char *>(rows.images) +
+		first * rows.in_rows.s_outer * static_cast<int64_t>(rows.element_bytes);
+	const float *scales = rows.scales != nullptr ? rows.scales + first * rows.scale_stride : nullptr;
+	launch_rfft_rows(rows.kind, images, in, rows.m, scales, rows.scale_stride,
+		columns.spectra + first * rows.out_rows.s_outer, out, rows.n, stream);
+}
+
 // Host entry of every insertion: fills the workspace and launches.
 void run_insertion(insertion_params p, int schedule, const float2 *num_in, const int64_t *num_in_strides,
 	const float *den_in, const int64_t *den_in_strides, bool in_place, double shape, const pending_columns *columns,
@@ -1627,6 +1646,14 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 	// Pending column transforms run inside the preparation of each chunk on
 	// the brick schedule; on the scatter schedule they run here, in place.
 	const bool fused_columns = columns != nullptr && brick;
+	// Pending row transforms run per chunk (just before the chunk's column
+	// pass, so a chunk's spectra are still in the caches) when every chunk is
+	// a range of the images (one volume), else here for all images.
+	const bool chunk_rows = fused_columns && columns->rows != nullptr && p.map.kept_count == 1;
+	if (columns != nullptr && columns->rows != nullptr && !chunk_rows)
+	{
+		launch_rows(*columns, 0, columns->rows->in_rows.count / columns->rows->m, stream);
+	}
 	if (columns != nullptr && !fused_columns && columns->lines.count > 0)
 	{
 		launch_fft_lines(columns->spectra, columns->lines, columns->spectra, columns->lines, p.n, -1.0f, 1.0f,
@@ -1644,6 +1671,10 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 		for (int64_t begin = 0; begin < p.map.reduced_count; begin += chunk_count)
 		{
 			const slice_chunk chunk{begin, std::min(chunk_count, p.map.reduced_count - begin)};
+			if (chunk_rows)
+			{
+				launch_rows(*columns, chunk.begin, chunk.begin + chunk.count, stream);
+			}
 			const int prepare_token = timing_begin(REXLIB_HIP_KERNEL_PREPARE, stream);
 			if (fused_columns)
 			{

@@ -66,13 +66,32 @@ enum insertion_schedule : int
 	schedule_brick = 2
 };
 
+// Real images whose row transforms are still to run (the fused image
+// insertion): image b has `m` rows of `in_rows` from element b * s_outer of
+// `images`, and its n rows of half spectra go to `out_rows` of the spectra.
+struct pending_rows
+{
+	int kind; // real_kind
+	const void *images;
+	std::size_t element_bytes;
+	line_set in_rows;
+	int m;
+	const float *scales; // per-image scales of float16 images, or null
+	int64_t scale_stride;
+	line_set out_rows;
+	int64_t n;
+};
+
 // Slices whose rows are transformed but whose columns are not yet (the
 // fused image insertion): the insertion runs the column transforms, on the
-// brick schedule together with the preparation of the samples.
+// brick schedule together with the preparation of the samples. With `rows`,
+// the row transforms are pending too, and the insertion runs them first
+// (per chunk of slices, on the brick schedule).
 struct pending_columns
 {
 	float2 *spectra;
 	line_set lines;
+	const pending_rows *rows = nullptr;
 };
 
 // `columns`, when not null, describes the slices operand (transformed in

@@ -27,7 +27,7 @@ TABLE = os.path.join(HERE, "tuning.json")
 
 # (x, y, z) extents of the bricks the library is built for.
 SHAPES = ((16, 8, 6), (16, 8, 8), (16, 16, 8), (8, 8, 8), (8, 8, 6), (8, 8, 4), (8, 6, 6))
-SHARES = (64, 128, 256, 512, 1024)
+SHARES = (32, 64, 128, 256, 512, 1024)
 DEFAULT = {"extent_x": 16, "extent_y": 8, "extent_z": 6, "share": 256}
 
 

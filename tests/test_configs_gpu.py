@@ -98,7 +98,7 @@ def test_full_launch_of_256_images_into_256_cubed(tuning_table):
 
 
 BOXES = [64, 96, 128, 192, 256, 320]
-SHARES = [64, 128, 1024]
+SHARES = [32, 128, 1024]
 
 
 def random_ctf_case(n, big, count, seed):
