@@ -10,7 +10,7 @@ import torch
 
 import oracle
 from gpu_helpers import to_device
-from rexlib_amd import batched, synthetic
+from rexlib_amd import batched, em, synthetic
 
 pytestmark = pytest.mark.gpu
 
@@ -65,3 +65,31 @@ def test_one_batched_reconstruction_matches_the_oracle():
                             oracle.Window.linear(), pixel_size=1.5)
     assert oracle.rel_l2(r.numerator.cpu().numpy(), ref[0]) <= 1e-4
     assert oracle.rel_l2(r.denominator.cpu().numpy(), ref[1]) <= 1e-4
+
+
+def test_replay_after_a_larger_eager_call_on_the_capture_streams():
+    # A graph holds the workspace addresses of its capture. A larger eager
+    # insertion on the same streams afterwards grows their workspaces; the
+    # areas the graph holds must stay valid (they are retired, never freed),
+    # so a later replay still equals the sequential run.
+    recons = make_recons(seed=11)
+    batched.run_sequential(recons, tuning=False)
+    torch.cuda.synchronize()
+    expected = snapshot(recons)
+    b = batched.BatchedReconstruction(recons, streams=2, tuning=False).capture()
+    big = make_recons(seed=12)
+    bigger = batched.Reconstruction(torch.cat([big[3].images] * 4), torch.cat([big[3].transforms] * 4),
+                                    torch.cat([big[3].ctf] * 4))
+    bigger.allocate()
+    for stream in b.streams:
+        with torch.cuda.stream(stream):
+            batched.run_one(bigger, em.ExecutionContext(stream=stream))
+    torch.cuda.synchronize()
+    for r in recons:
+        r.numerator.fill_(float("nan"))
+    b.replay()
+    torch.cuda.synchronize()
+    for (num, den, vol), r in zip(expected, recons):
+        assert oracle.rel_l2(r.numerator.cpu().numpy(), num.cpu().numpy()) < 1e-5
+        assert oracle.rel_l2(r.denominator.cpu().numpy(), den.cpu().numpy()) < 1e-5
+        assert oracle.rel_l2(r.volume.cpu().numpy(), vol.cpu().numpy()) < 1e-4

@@ -256,6 +256,19 @@ def test_window_transforms_at_known_points():
     # triangle: sinc^2(1/4) at x - C = 2 of N = 8
     assert oracle.window_transform(oracle.Window.linear(), 8)[6] == pytest.approx(
         (math.sin(math.pi / 4) / (math.pi / 4)) ** 2)
+    # box (nearest): sinc(1/4) = 2 sqrt(2) / pi at x - C = 2 of N = 8, and the
+    # first zero of sinc at x - C = N (outside the volume), so sinc(-1/2) at
+    # index 0 of N = 4 is 2 / pi.
+    box = oracle.window_transform(oracle.Window.nearest(), 8)
+    assert box[4] == 1.0
+    assert box[6] == pytest.approx(2 * math.sqrt(2) / math.pi)
+    assert box[2] == box[6]
+    assert oracle.window_transform(oracle.Window.nearest(), 4)[0] == pytest.approx(2 / math.pi)
+    # The box transform equals a direct numerical transform of the box of width 1.
+    d = np.linspace(-0.5, 0.5, 2001)
+    for j in (5, 7):
+        u = (j - 4) / 8
+        assert box[j] == pytest.approx(np.trapezoid(np.cos(2 * math.pi * u * d), d), rel=1e-6)
     kb = oracle.window_transform(oracle.Window.kaiser_bessel(2.0, 8.0), 64)
     assert kb[32] == pytest.approx(1.0)
     # The KB transform equals a direct numerical transform of the window.

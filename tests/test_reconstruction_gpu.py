@@ -108,3 +108,36 @@ def test_plain_cpu_configuration_end_to_end(count):
     assert oracle.rel_l2(den.cpu().numpy(), ref[1]) <= 1e-4
     ref_volume = oracle.irfft3(oracle.normalize(ref[0].astype(np.complex128), ref[1].astype(np.float64)), big)
     assert oracle.rel_l2(volume.cpu().numpy(), ref_volume) <= 1e-4
+
+
+def test_blob_stack_reconstructs_its_ground_truth():
+    # The generator against its own ground truth: 6,000 exact projections of
+    # a 64^3 blob volume (CTF with defocus 1-3 um, SNR 10), fused rfft2 + CTF
+    # insertion, quotient, irfft3 and gridding correction, compared with
+    # BlobVolume.render(). A sign or axis convention of the projector or of
+    # the CTF that disagreed with the insertion would turn the correlation
+    # negative or scramble the blobs.
+    big = 64
+    vol = synthetic.blob_volume(big, 30, big * 0.3, (2.0, 4.0), seed=21)
+    stack = synthetic.particle_stack(vol, 6000, big, seed=22, snr=10.0, device="cuda")
+    num = torch.zeros((big, big, big // 2 + 1), dtype=torch.complex64, device="cuda")
+    den = torch.zeros((big, big, big // 2 + 1), dtype=torch.float32, device="cuda")
+    window = em.GriddingWindow.linear()
+    em.insert_ctf_images(num, den, stack.images, to_device(stack.transforms), to_device(stack.ctf),
+                         torch.ones(big // 2 + 1, device="cuda"), window, OPTICS, 1.5)
+    volume = rexlib_amd.irfft3(em.normalize(num, den, 1e-3))
+    volume = em.gridding_correct(volume, window, out=volume).double()
+    truth = vol.render(device="cuda", dtype=torch.float64)
+    c = big // 2
+    r = torch.arange(big, device="cuda", dtype=torch.float64) - c
+    inside = (r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2) <= (0.4 * big) ** 2
+    a = volume[inside] - volume[inside].mean()
+    b = truth[inside] - truth[inside].mean()
+    corr = float((a * b).sum() / torch.sqrt((a * a).sum() * (b * b).sum()))
+    fsc = em.fsc(torch.fft.rfftn(volume.float()).to(torch.complex64).contiguous(),
+                 torch.fft.rfftn(truth.float()).to(torch.complex64).contiguous()).cpu().numpy()
+    print("real-space correlation", corr, "FSC shells 1-10", np.round(fsc[1:11], 4))
+    # Measured: correlation 0.987; FSC 0.90-0.997 over shells 1-10, lowest at
+    # shell 2, where the CTF is small and the SNR of the data with it.
+    assert corr > 0.95
+    assert fsc[1:11].min() > 0.8
