@@ -1380,8 +1380,18 @@ void dispatch_columns_prepare(const insertion_params &p, slice_chunk chunk, prep
 }
 
 // The tasks of a brick grid as (packed box, (share, shares)) pairs,
-// heaviest first, for slice counts up to `bucket`.
-std::vector<int4> brick_task_items(int big, int64_t bucket, int target, int ex, int ey, int ez, int taps)
+// heaviest first, for slice counts up to `bucket`, on a device that runs
+// `slots` brick wavefronts at once.
+//
+// A plane through the origin meets a brick at distance r with probability
+// about min(1, 13 / r), and the samples it leaves there grow as the brick
+// nears the origin (the plane's section of the brick widens), about as
+// 1 / r down to the brick's half diagonal. The expected load of a brick is
+// taken as bucket * 13 / max(r, half diagonal) plane hits, and a brick is
+// split into shares of at most `target` of them -- or fewer, so that no task
+// carries more than the average load of a slot (the total over `slots`):
+// a task longer than that would outlast the rest of the launch.
+std::vector<int4> brick_task_items(int big, int64_t bucket, int target, int ex, int ey, int ez, int taps, int slots)
 {
 	const int half = big / 2;
 	// First-tap cells: x in [-(taps - 1), N/2]; y, z in [-(N/2) - (taps - 1),
@@ -1409,7 +1419,14 @@ std::vector<int4> brick_task_items(int big, int64_t bucket, int target, int ex, 
 		int4 share;
 		double load;
 	};
-	std::vector<item> items;
+	struct brick
+	{
+		int4 box;
+		double load;
+	};
+	std::vector<brick> bricks;
+	double total = 0.0;
+	const double half_diagonal = 0.5 * std::sqrt(static_cast<double>(ex * ex + ey * ey + ez * ez));
 	for (const auto &z : starts(-half - reach, half, ez))
 	{
 		for (const auto &y : starts(-half - reach, half, ey))
@@ -1422,16 +1439,20 @@ std::vector<int4> brick_task_items(int big, int64_t bucket, int target, int ex, 
 				const double cy = y.first + 0.5 * (y.second - 1);
 				const double cz = z.first + 0.5 * (z.second - 1);
 				const double r = std::sqrt(cx * cx + cy * cy + cz * cz);
-				// A plane through the origin passes a brick at distance r with
-				// probability about min(1, 13 / r).
-				const double hits = static_cast<double>(bucket) * std::min(1.0, 13.0 / std::max(r, 1.0));
-				const int shares = static_cast<int>(std::min<double>(64.0, std::max(1.0, std::ceil(hits / target))));
-				for (int c = 0; c < shares; ++c)
-				{
-					items.push_back({make_int4(x.first, y.first, z.first, w), make_int4(c, shares, 0, 0),
-						hits / shares});
-				}
+				const double load = static_cast<double>(bucket) * 13.0 / std::max(r, std::max(half_diagonal, 1.0));
+				bricks.push_back({make_int4(x.first, y.first, z.first, w), load});
+				total += load;
 			}
+		}
+	}
+	const double limit = std::min<double>(target, std::max(16.0, total / std::max(slots, 1)));
+	std::vector<item> items;
+	for (const brick &b : bricks)
+	{
+		const int shares = static_cast<int>(std::min<double>(64.0, std::max(1.0, std::ceil(b.load / limit))));
+		for (int c = 0; c < shares; ++c)
+		{
+			items.push_back({b.box, make_int4(c, shares, 0, 0), b.load / shares});
 		}
 	}
 	std::stable_sort(items.begin(), items.end(), [](const item &a, const item &b) { return a.load > b.load; });
@@ -1470,8 +1491,13 @@ task_list &brick_task_list(int big, int64_t slices, const brick_tuning &tuning, 
 	auto found = task_cache.find(key);
 	if (found == task_cache.end())
 	{
+		// Brick wavefronts resident at once: 16 per CU (the LDS of a brick
+		// admits no more on gfx950).
+		int units = 0;
+		REXLIB_HIP_CHECK(hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device));
 		found = task_cache.emplace(key, task_list{}).first;
-		found->second.host = brick_task_items(big, bucket, tuning.share, tuning.ex, tuning.ey, tuning.ez, taps);
+		found->second.host = brick_task_items(big, bucket, tuning.share, tuning.ex, tuning.ey, tuning.ez, taps,
+			16 * units);
 	}
 	return found->second;
 }
