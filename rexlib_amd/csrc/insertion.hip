@@ -839,9 +839,10 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 		int held_cell = 0;
 		float held_w[3] = {0.0f, 0.0f, 0.0f};
 
-		// One round: free lanes take the next samples of the current rows
-		// (when `fill`), then every holding lane claims its cell and the
-		// winners add their taps.
+		// One round: every holding lane claims its cell and the winners add
+		// their taps; meanwhile (when `fill`) the lanes free after the round
+		// load the next samples of the current rows, which they hold from the
+		// next round on.
 		int next = 0;
 		int total = 0;
 		int current = 0;
@@ -849,10 +850,40 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 		int row_count = 0;
 		auto round = [&](bool fill)
 		{
+			// Claim: each holding lane writes its lane into the byte of its
+			// first-tap cell and reads it back. A plain LDS read after the
+			// store (lds_order keeps the compiler from reusing the stored
+			// value); a volatile pointer would turn it into a flat load, which
+			// is not ordered after the LDS store.
+			const int lx = held_cell & 0xff;
+			const int ly = (held_cell >> 8) & 0xff;
+			const int lz = (held_cell >> 16) & 0xff;
+			const int cell = lx + EX * (ly + EY * lz);
+			if (pending)
+			{
+				claim[cell] = static_cast<unsigned char>(lane);
+			}
+			lds_order();
+			const bool win = pending && claim[cell] == lane;
+			lds_order();
+
+			// The fill starts before the winners' taps: the lanes free after
+			// this round (holding nothing, or winning now) take the next
+			// samples of the current rows, and the loads of those samples are
+			// in flight while the taps run. The samples join the claim of the
+			// next round.
+			int take = 0;
+			uint64_t free_mask = 0;
+			int r = 0;
+			float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			float4 st = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			uint32_t ix = 0;
+			int kx = 0;
+			prepared_sample q{0.0f, 0.0f, 0.0f};
 			if (fill)
 			{
-				const uint64_t free_mask = __ballot(!pending);
-				const int take = min(static_cast<int>(__popcll(free_mask)), total - next);
+				free_mask = __ballot(!pending || win);
+				take = min(static_cast<int>(__popcll(free_mask)), total - next);
 				if (take > 0)
 				{
 					// The row of sample next + lane: rows starting in [next, next + 64)
@@ -864,78 +895,19 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 						marker[row_start - next] = lane;
 					}
 					lds_order();
-					const int r = wave_prefix_max(marker[lane]);
+					r = wave_prefix_max(marker[lane]);
 					lds_order();
-					const float4 b = row_base[r];
-					const float4 st = row_step[r];
-					const uint32_t ix = row_index[r];
-					const int kx = __float_as_int(st.w) + (next + lane - __float_as_int(b.w));
-					const bool make = lane < take;
-					prepared_sample q{0.0f, 0.0f, 0.0f};
-					if (make)
+					b = row_base[r];
+					st = row_step[r];
+					ix = row_index[r];
+					kx = __float_as_int(st.w) + (next + lane - __float_as_int(b.w));
+					if (lane < take)
 					{
 						q = prepared[(ix & 0x7fffffffu) + static_cast<uint32_t>(kx)];
 					}
-					const float c[3] = {row_position(b.x, st.x, kx), row_position(b.y, st.y, kx),
-						row_position(b.z, st.z, kx)};
-					int local[3];
-					float w[3];
-#pragma unroll
-					for (int a = 0; a < 3; ++a)
-					{
-						const int f = window_taps<PROFILE>::first(p, c[a]);
-						local[a] = f - box.lo[a];
-						w[a] = PROFILE == REXLIB_HIP_TRIANGLE ? c[a] - static_cast<float>(f) : c[a];
-					}
-					const bool own = static_cast<unsigned>(local[0]) < static_cast<unsigned>(box.ext[0]) &&
-						static_cast<unsigned>(local[1]) < static_cast<unsigned>(box.ext[1]) &&
-						static_cast<unsigned>(local[2]) < static_cast<unsigned>(box.ext[2]);
-					const bool value = !(q.re == 0.0f && q.im == 0.0f && q.den == 0.0f);
-					const int cell = make && own && value ?
-						(local[0] | (local[1] << 8) | (local[2] << 16) | (1 << 24)) : 0;
-					const float im = (ix >> 31) != 0 ? -q.im : q.im;
-					current = __builtin_amdgcn_readlane(r, take - 1);
-					// Free lane k (in lane order) takes sample next + k.
-					const int rank = static_cast<int>(__popcll(free_mask & below));
-					const int source = rank < take ? rank : 0;
-					const int got = __shfl(cell, source, 64);
-					const float got_re = __shfl(q.re, source, 64);
-					const float got_im = __shfl(im, source, 64);
-					const float got_den = __shfl(q.den, source, 64);
-					float got_w[3];
-#pragma unroll
-					for (int a = 0; a < 3; ++a)
-					{
-						got_w[a] = __shfl(w[a], source, 64);
-					}
-					if (!pending && rank < take && (got >> 24) != 0)
-					{
-						pending = true;
-						held_re = got_re;
-						held_im = got_im;
-						held_den = got_den;
-						held_cell = got;
-						held_w[0] = got_w[0];
-						held_w[1] = got_w[1];
-						held_w[2] = got_w[2];
-					}
-					next += take;
 				}
 			}
-			const int lx = held_cell & 0xff;
-			const int ly = (held_cell >> 8) & 0xff;
-			const int lz = (held_cell >> 16) & 0xff;
-			const int cell = lx + EX * (ly + EY * lz);
-			// A plain LDS read after the store (lds_order keeps the compiler
-			// from reusing the stored value); a volatile pointer would turn it
-			// into a flat load, which is not ordered after the LDS store.
-			if (pending)
-			{
-				claim[cell] = static_cast<unsigned char>(lane);
-			}
-			lds_order();
-			const bool win = pending && claim[cell] == lane;
-			lds_order();
+
 			if (win)
 			{
 				const int base = lx + L::pitch_y * ly + L::pitch_z * lz;
@@ -988,6 +960,54 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 			}
 			lds_order();
 			pending = pending && !win;
+
+			if (take > 0)
+			{
+				const float c[3] = {row_position(b.x, st.x, kx), row_position(b.y, st.y, kx),
+					row_position(b.z, st.z, kx)};
+				int local[3];
+				float w[3];
+#pragma unroll
+				for (int a = 0; a < 3; ++a)
+				{
+					const int f = window_taps<PROFILE>::first(p, c[a]);
+					local[a] = f - box.lo[a];
+					w[a] = PROFILE == REXLIB_HIP_TRIANGLE ? c[a] - static_cast<float>(f) : c[a];
+				}
+				const bool own = static_cast<unsigned>(local[0]) < static_cast<unsigned>(box.ext[0]) &&
+					static_cast<unsigned>(local[1]) < static_cast<unsigned>(box.ext[1]) &&
+					static_cast<unsigned>(local[2]) < static_cast<unsigned>(box.ext[2]);
+				const bool value = !(q.re == 0.0f && q.im == 0.0f && q.den == 0.0f);
+				const int got_cell = lane < take && own && value ?
+					(local[0] | (local[1] << 8) | (local[2] << 16) | (1 << 24)) : 0;
+				const float im = (ix >> 31) != 0 ? -q.im : q.im;
+				current = __builtin_amdgcn_readlane(r, take - 1);
+				// Free lane k (in lane order) takes sample next + k.
+				const int rank = static_cast<int>(__popcll(free_mask & below));
+				const int source = rank < take ? rank : 0;
+				const int got = __shfl(got_cell, source, 64);
+				const float got_re = __shfl(q.re, source, 64);
+				const float got_im = __shfl(im, source, 64);
+				const float got_den = __shfl(q.den, source, 64);
+				float got_w[3];
+#pragma unroll
+				for (int a = 0; a < 3; ++a)
+				{
+					got_w[a] = __shfl(w[a], source, 64);
+				}
+				if (!pending && rank < take && (got >> 24) != 0)
+				{
+					pending = true;
+					held_re = got_re;
+					held_im = got_im;
+					held_den = got_den;
+					held_cell = got;
+					held_w[0] = got_w[0];
+					held_w[1] = got_w[1];
+					held_w[2] = got_w[2];
+				}
+				next += take;
+			}
 		};
 
 		for (int64_t window = share_begin; window < share_end; window += 64)
