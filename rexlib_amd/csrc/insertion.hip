@@ -780,12 +780,6 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 	__shared__ brick_voxel acc[L::voxels];
 	// Per first-tap cell: the lane that last claimed it.
 	__shared__ unsigned char claim[L::cells];
-	// The current 64 rows: base position and first sample number; step and
-	// first k_x; the prepared element of (row, k_x = 0), bit 31 set for the
-	// -k copy.
-	__shared__ float4 row_base[64];
-	__shared__ float4 row_step[64];
-	__shared__ uint32_t row_index[64];
 	// Per lane: the last row (or slice) starting at or before it.
 	__shared__ int marker[64];
 
@@ -848,6 +842,13 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 		int current = 0;
 		int row_start = 0;
 		int row_count = 0;
+		// Lane k describes row k of the current 64 rows (registers, not LDS,
+		// so that more wavefronts fit a CU): base position and first sample
+		// number; step and first k_x; the prepared element of (row, k_x = 0),
+		// bit 31 set for the -k copy.
+		float4 row_base = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		float4 row_step = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		uint32_t row_index = 0;
 		auto round = [&](bool fill)
 		{
 			// Claim: each holding lane writes its lane into the byte of its
@@ -897,9 +898,11 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 					lds_order();
 					r = wave_prefix_max(marker[lane]);
 					lds_order();
-					b = row_base[r];
-					st = row_step[r];
-					ix = row_index[r];
+					b = make_float4(__shfl(row_base.x, r, 64), __shfl(row_base.y, r, 64), __shfl(row_base.z, r, 64),
+						__shfl(row_base.w, r, 64));
+					st = make_float4(__shfl(row_step.x, r, 64), __shfl(row_step.y, r, 64), __shfl(row_step.z, r, 64),
+						__shfl(row_step.w, r, 64));
+					ix = static_cast<uint32_t>(__shfl(static_cast<int>(row_index), r, 64));
 					kx = __float_as_int(st.w) + (next + lane - __float_as_int(b.w));
 					if (lane < take)
 					{
@@ -1134,9 +1137,9 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 				row_start = count_inclusive - row_count;
 				total = __builtin_amdgcn_readlane(count_inclusive, 63);
 				const int stored_row = ky < 0 ? ky + n : ky;
-				row_base[lane] = make_float4(base[0], base[1], base[2], __int_as_float(row_start));
-				row_step[lane] = make_float4(step[0], step[1], step[2], __int_as_float(begin));
-				row_index[lane] = r < total_rows ? (static_cast<uint32_t>((kept * chunk.count + js) * elements +
+				row_base = make_float4(base[0], base[1], base[2], __int_as_float(row_start));
+				row_step = make_float4(step[0], step[1], step[2], __int_as_float(begin));
+				row_index = r < total_rows ? (static_cast<uint32_t>((kept * chunk.count + js) * elements +
 					static_cast<int64_t>(stored_row) * h) | (minus ? 0x80000000u : 0u)) : 0u;
 				lds_order();
 				next = 0;
