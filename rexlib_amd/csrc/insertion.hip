@@ -1514,13 +1514,19 @@ task_list &brick_task_list(int big, int64_t slices, const brick_tuning &tuning, 
 	auto found = task_cache.find(key);
 	if (found == task_cache.end())
 	{
-		// Brick wavefronts resident at once: 16 per CU (the LDS of a brick
-		// admits no more on gfx950).
+		// Brick wavefronts resident at once: as many per CU as its 160 KiB of
+		// LDS holds (the brick's voxels, claim bytes and 64 markers, in
+		// 512-byte granules).
 		int units = 0;
 		REXLIB_HIP_CHECK(hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device));
+		const int lx = tuning.ex + taps - 1, ly = tuning.ey + taps - 1, lz = tuning.ez + taps - 1;
+		const int pitch_y = lx % 2 == 0 ? lx + 1 : lx;
+		const int pitch_z = (pitch_y * ly) % 2 == 0 ? pitch_y * ly + 1 : pitch_y * ly;
+		const int lds = static_cast<int>(align_up(static_cast<std::size_t>(pitch_z * lz * 12 +
+			tuning.ex * tuning.ey * tuning.ez + 64 * 4), 512));
 		found = task_cache.emplace(key, task_list{}).first;
 		found->second.host = brick_task_items(big, bucket, tuning.share, tuning.ex, tuning.ey, tuning.ez, taps,
-			16 * units);
+			units * std::max(1, (160 * 1024) / lds));
 	}
 	return found->second;
 }

@@ -293,6 +293,17 @@ def test_normalize_threshold():
     np.testing.assert_allclose(oracle.normalize(num, den), [0.5 + 0.5j, 0, 6])
 
 
+def test_shell_index_counts_lattice_points_by_hand():
+    # Shell s holds the points with s - 1/2 <= |g| < s + 1/2. On the half grid
+    # (x >= 0) of N = 16: shell 0 is the origin; shell 1 is |g|^2 in {1, 2}:
+    # 5 axis points and 8 face diagonals with x >= 0 (18 in the full grid);
+    # shell 2 is |g|^2 in {3, 4, 5, 6}: 4 + 5 + 16 + 12 points with x >= 0.
+    s = oracle.shell_index(16)
+    assert [(s == k).sum() for k in range(3)] == [1, 13, 37]
+    assert s[0, 0, 1] == 1 and s[1, 1, 1] == 2 and s[0, 2, 1] == 2  # |g| = 1, sqrt(3), sqrt(5)
+    assert s[8, 0, 0] == 8 and s[8, 8, 8] == 14  # the Nyquist index is -8: |g| = 8, 8 sqrt(3)
+
+
 def test_shell_is_round_half_up_of_the_radius_by_the_integer_walk():
     # The reference's walk (row_insertion_kernel.inl:180-220): the largest s
     # with 4 s^2 - 4 s + 1 <= 4 |k|^2; equal to floor(|k| + 1/2), ties up.
