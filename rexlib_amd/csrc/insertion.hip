@@ -874,8 +874,8 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 			// in flight while the taps run. The samples join the claim of the
 			// next round.
 			int take = 0;
-			uint64_t free_mask = 0;
 			int r = 0;
+			bool mine = false;
 			float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 			float4 st = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 			uint32_t ix = 0;
@@ -883,7 +883,8 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 			prepared_sample q{0.0f, 0.0f, 0.0f};
 			if (fill)
 			{
-				free_mask = __ballot(!pending || win);
+				const bool free = !pending || win;
+				const uint64_t free_mask = __ballot(free);
 				take = min(static_cast<int>(__popcll(free_mask)), total - next);
 				if (take > 0)
 				{
@@ -898,13 +899,19 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 					lds_order();
 					r = wave_prefix_max(marker[lane]);
 					lds_order();
-					b = make_float4(__shfl(row_base.x, r, 64), __shfl(row_base.y, r, 64), __shfl(row_base.z, r, 64),
-						__shfl(row_base.w, r, 64));
-					st = make_float4(__shfl(row_step.x, r, 64), __shfl(row_step.y, r, 64), __shfl(row_step.z, r, 64),
-						__shfl(row_step.w, r, 64));
-					ix = static_cast<uint32_t>(__shfl(static_cast<int>(row_index), r, 64));
-					kx = __float_as_int(st.w) + (next + lane - __float_as_int(b.w));
-					if (lane < take)
+					current = __builtin_amdgcn_readlane(r, take - 1);
+					// Free lane k (in lane order) takes sample next + k itself: the
+					// row of that sample, then the row's data, by shuffles.
+					const int rank = static_cast<int>(__popcll(free_mask & below));
+					mine = free && rank < take;
+					const int row = __shfl(r, mine ? rank : 0, 64);
+					b = make_float4(__shfl(row_base.x, row, 64), __shfl(row_base.y, row, 64),
+						__shfl(row_base.z, row, 64), __shfl(row_base.w, row, 64));
+					st = make_float4(__shfl(row_step.x, row, 64), __shfl(row_step.y, row, 64),
+						__shfl(row_step.z, row, 64), __shfl(row_step.w, row, 64));
+					ix = static_cast<uint32_t>(__shfl(static_cast<int>(row_index), row, 64));
+					kx = __float_as_int(st.w) + (next + rank - __float_as_int(b.w));
+					if (mine)
 					{
 						q = prepared[(ix & 0x7fffffffu) + static_cast<uint32_t>(kx)];
 					}
@@ -981,33 +988,16 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 					static_cast<unsigned>(local[1]) < static_cast<unsigned>(box.ext[1]) &&
 					static_cast<unsigned>(local[2]) < static_cast<unsigned>(box.ext[2]);
 				const bool value = !(q.re == 0.0f && q.im == 0.0f && q.den == 0.0f);
-				const int got_cell = lane < take && own && value ?
-					(local[0] | (local[1] << 8) | (local[2] << 16) | (1 << 24)) : 0;
-				const float im = (ix >> 31) != 0 ? -q.im : q.im;
-				current = __builtin_amdgcn_readlane(r, take - 1);
-				// Free lane k (in lane order) takes sample next + k.
-				const int rank = static_cast<int>(__popcll(free_mask & below));
-				const int source = rank < take ? rank : 0;
-				const int got = __shfl(got_cell, source, 64);
-				const float got_re = __shfl(q.re, source, 64);
-				const float got_im = __shfl(im, source, 64);
-				const float got_den = __shfl(q.den, source, 64);
-				float got_w[3];
-#pragma unroll
-				for (int a = 0; a < 3; ++a)
-				{
-					got_w[a] = __shfl(w[a], source, 64);
-				}
-				if (!pending && rank < take && (got >> 24) != 0)
+				if (mine && own && value)
 				{
 					pending = true;
-					held_re = got_re;
-					held_im = got_im;
-					held_den = got_den;
-					held_cell = got;
-					held_w[0] = got_w[0];
-					held_w[1] = got_w[1];
-					held_w[2] = got_w[2];
+					held_re = q.re;
+					held_im = (ix >> 31) != 0 ? -q.im : q.im;
+					held_den = q.den;
+					held_cell = local[0] | (local[1] << 8) | (local[2] << 16) | (1 << 24);
+					held_w[0] = w[0];
+					held_w[1] = w[1];
+					held_w[2] = w[2];
 				}
 				next += take;
 			}
