@@ -659,10 +659,19 @@ __global__ __launch_bounds__(256) void columns_prepare_kernel(insertion_params p
 // bound_rows of one slice and sign: the rows [r0, r1) whose samples k_x in
 // [first, widest] can reach the box [lower, upper]; r0 >= r1 when none.
 // Evaluated per lane, one slice per lane.
-// `inv_y` holds 1 / y_step_f per axis (0 where the step is 0); a product with
-// it stands for the division, the box being widened far beyond its rounding.
+// `inv_y` and `inv_x` hold 1 / y_step_f and 1 / x_step per axis (0 where the
+// step is 0); a product with them stands for the division, the box being
+// widened far beyond its rounding.
+//
+// The reference bounds each axis separately over the whole k_x range
+// (row_interval.cpp:34-136), which keeps rows whose line meets the box only
+// at k_x the slice does not hold. Here the bound is refined once: the k_x
+// (integers) that can reach the box from those rows, then the rows again
+// for that k_x range. Each step keeps every row holding a sample in the
+// box, and about half the rows go.
 __device__ inline void bound_rows(const insertion_params &p, const float *x_step, const float *y_step_f,
-	const float *inv_y, int last_shell, const float *lower, const float *upper, int sign_index, int &r0, int &r1)
+	const float *inv_y, const float *inv_x, int last_shell, const float *lower, const float *upper, int sign_index,
+	int &r0, int &r1)
 {
 	r0 = 0;
 	r1 = 0;
@@ -705,6 +714,48 @@ __device__ inline void bound_rows(const insertion_params &p, const float *x_step
 		}
 	}
 	if (none || low > high)
+	{
+		return;
+	}
+	float k_low = static_cast<float>(first);
+	float k_high = static_cast<float>(widest);
+	for (int a = 0; a < 3; ++a)
+	{
+		const float step = sign * x_step[a];
+		if (step != 0.0f)
+		{
+			const float y_step = sign * y_step_f[a];
+			const float b0 = low * y_step;
+			const float b1 = high * y_step;
+			const float inv = sign * inv_x[a];
+			const float l = (lower[a] - fmaxf(b0, b1)) * inv;
+			const float u = (upper[a] - fminf(b0, b1)) * inv;
+			k_low = fmaxf(k_low, fminf(l, u));
+			k_high = fminf(k_high, fmaxf(l, u));
+		}
+	}
+	k_low = ceilf(k_low);
+	k_high = floorf(k_high);
+	if (k_low > k_high)
+	{
+		return;
+	}
+	for (int a = 0; a < 3; ++a)
+	{
+		const float step = sign * x_step[a];
+		const float y_step = sign * y_step_f[a];
+		if (y_step != 0.0f)
+		{
+			const float at_first = k_low * step;
+			const float at_last = k_high * step;
+			const float inv = sign * inv_y[a];
+			const float l = (lower[a] - fmaxf(at_first, at_last)) * inv;
+			const float u = (upper[a] - fminf(at_first, at_last)) * inv;
+			low = fmaxf(low, fminf(l, u));
+			high = fminf(high, fmaxf(l, u));
+		}
+	}
+	if (low > high)
 	{
 		return;
 	}
@@ -1055,8 +1106,9 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 				ix0 = x0 == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(x0);
 				ix1 = x1 == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(x1);
 				ix2 = x2 == 0.0f ? 0.0f : __builtin_amdgcn_rcpf(x2);
-				bound_rows(p, xs, ys, iy, last_shell, lower, upper, 0, r0_plus, r1_plus);
-				bound_rows(p, xs, ys, iy, last_shell, lower, upper, 1, r0_minus, r1_minus);
+				const float ixs[3] = {ix0, ix1, ix2};
+				bound_rows(p, xs, ys, iy, ixs, last_shell, lower, upper, 0, r0_plus, r1_plus);
+				bound_rows(p, xs, ys, iy, ixs, last_shell, lower, upper, 1, r0_minus, r1_minus);
 			}
 			const int plus_rows = near ? max(r1_plus - r0_plus, 0) : 0;
 			const int minus_rows = near ? max(r1_minus - r0_minus, 0) : 0;
