@@ -23,8 +23,8 @@ Reconstruction with weights W: V = gridding_correct(irfft3(numerator * W)).
 """
 from __future__ import annotations
 
-import ctypes
 import math
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -230,14 +230,12 @@ def reconstruct(numerator: torch.Tensor, weights: torch.Tensor, window=None) -> 
     return em.gridding_correct(volume, window, out=volume)
 
 
-MODEL_FILE = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)),
-                                         "compensation_model.npz")
+MODEL_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "compensation_model.npz")
 
 
 def shipped_model(device="cuda") -> CompensationModel | None:
     """The model trained by ``rexlib_amd.train_compensation`` and shipped with
     the package, or None when the file is absent."""
-    import os
     return CompensationModel.load(MODEL_FILE, device) if os.path.exists(MODEL_FILE) else None
 
 
