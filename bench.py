@@ -474,6 +474,30 @@ def batched_config(torch, count=64, seed=5000, low=500, high=40000):
     return out
 
 
+def plugin_path(args):
+    """The rexlib call sequence through the plugin itself, as a rexlib caller
+    would run it: tests/plugin_driver (a C++ rexlib caller) loads
+    librexlib-hip.so and times rfft2 + insert_ctf_slices per launch, then
+    irfft3, over the same image count and launch size (256^2 images into
+    256^3; its own random images and orientations). None for other shapes
+    or when the driver is not built."""
+    import subprocess
+    driver = os.path.join(ROOT, "tests", "plugin_driver")
+    plugin = os.path.join(ROOT, "rexlib_amd", "rexlib-plugins", "librexlib-hip.so")
+    if args.n != 256 or args.big != 256 or not (os.path.exists(driver) and os.path.exists(plugin)):
+        return None
+    proc = subprocess.run([driver, "bench", plugin, str(args.images), str(args.launch), "2"], capture_output=True,
+                          text=True, timeout=600)
+    lines = [line for line in proc.stdout.splitlines() if line.startswith("{")]
+    if proc.returncode != 0 or not lines:
+        log(f"plugin path failed ({proc.returncode}): {proc.stdout[-400:]} {proc.stderr[-400:]}")
+        return None
+    out = json.loads(lines[-1])
+    out["calls"] = "through librexlib-hip.so: rfft2 + insert_ctf_slices per launch, then irfft3"
+    log(f"plugin path: {out}")
+    return out
+
+
 def parity_sample(args, stack, torch, em, rexlib_amd, count=256):
     """rel-L2 of the GPU accumulators (fused rfft2 + insertion, as timed) on the
     first `count` images against the reference's CPU path: pocketfft rfft2 in
@@ -522,7 +546,8 @@ def main():
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-parity", action="store_true")
     parser.add_argument("--no-forms", action="store_true",
-                        help="skip the precision, learned and batched legs reported beside the headline value")
+                        help="skip the precision, learned, batched and plugin-path legs reported beside the "
+                             "headline value")
     parser.add_argument("--no-config-forms", action="store_true",
                         help="skip the precision (200,000 128^2 images), batched (64 reconstructions) and learned-weight "
                              "(80,000 256^2 images) legs")
@@ -617,6 +642,7 @@ def main():
     two_call = {"images_per_s": args.images / (api_ms * 1e-3), "ms_per_step": api_ms,
                 "calls": "rfft2 + insert_ctf_slices per launch"}
     log(f"two-call API: {two_call}")
+    plugin = None if args.no_forms else plugin_path(args)
 
     # Average over launches; the final partial launch holds images % L images.
     images_timed = args.images * args.steps
@@ -688,6 +714,7 @@ def main():
         "cpu_baseline": baseline,
         "parity": parity,
         "two_call_api": two_call,
+        "plugin_path": plugin,
         "forms": forms,
     }
     print(json.dumps(line), flush=True)

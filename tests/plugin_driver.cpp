@@ -6,6 +6,9 @@
 //   plugin_driver info <plugin>            registration, compatibility and suitability (no GPU needed)
 //   plugin_driver run <plugin> <dir>       a reconstruction on hip:0 from the raw files in <dir>
 //   plugin_driver errors <plugin>          error behaviour on hip:0
+//   plugin_driver bench <plugin> <images> <launch> <steps>
+//                                          throughput of rfft2 + insert_ctf_slices per launch of
+//                                          256^2 images into 256^3, then irfft3, through the plugin
 //
 // Every line printed is a JSON object.
 #include "rexlib/em.hpp"
@@ -13,8 +16,11 @@
 
 #include <dlfcn.h>
 
+#include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <random>
 #include <fstream>
 #include <iostream>
 #include <map>
@@ -345,13 +351,117 @@ int errors(const std::string &path)
 	return 0;
 }
 
+// The benchmark's call sequence through the plugin: `images` 256^2 images in
+// launches of `launch` (the first `launch` random images reused by every
+// launch, with their own random rotations and CTF), each launch an rfft2 into
+// a spectra array and an insert_ctf_slices accumulating in place, then one
+// irfft3 of the numerator; one untimed step, then `steps` timed ones. The
+// accumulators are not cleared between steps (the work is the same).
+int bench(const std::string &path, std::size_t images, std::size_t launch, int steps)
+{
+	service_catalog catalog;
+	load_plugin(path, catalog);
+	const execution_context ctx = make_execution_context(catalog, device_index("hip", 0));
+	const std::size_t n = 256, big = 256, h = n / 2 + 1;
+	launch = std::min(launch, images);
+	std::mt19937_64 rng(2024);
+	std::normal_distribution<float> normal(0.0f, 1.0f);
+	std::uniform_real_distribution<double> uniform(0.0, 1.0);
+	array images_host = make_host_array({launch, n, n}, numerical_type::float32);
+	{
+		float *p = static_cast<float *>(images_host.get_buffer()->get_host_ptr());
+		for (std::size_t i = 0; i < launch * n * n; ++i)
+		{
+			p[i] = normal(rng);
+		}
+	}
+	array transforms_host = make_host_array({images, 3, 4}, numerical_type::float32);
+	array ctf_host = make_host_array({images, 4}, numerical_type::float32);
+	{
+		float *t = static_cast<float *>(transforms_host.get_buffer()->get_host_ptr());
+		float *c = static_cast<float *>(ctf_host.get_buffer()->get_host_ptr());
+		for (std::size_t i = 0; i < images; ++i)
+		{
+			// A uniform rotation from a random unit quaternion.
+			double q[4];
+			double norm = 0.0;
+			for (double &x : q)
+			{
+				x = std::sqrt(-2.0 * std::log(1.0 - uniform(rng))) * std::cos(2.0 * M_PI * uniform(rng));
+				norm += x * x;
+			}
+			norm = std::sqrt(norm);
+			const double a = q[0] / norm, b = q[1] / norm, cq = q[2] / norm, d = q[3] / norm;
+			const double r[9] = {a * a + b * b - cq * cq - d * d, 2 * (b * cq - a * d), 2 * (b * d + a * cq),
+				2 * (b * cq + a * d), a * a - b * b + cq * cq - d * d, 2 * (cq * d - a * b),
+				2 * (b * d - a * cq), 2 * (cq * d + a * b), a * a - b * b - cq * cq + d * d};
+			for (int row = 0; row < 3; ++row)
+			{
+				for (int col = 0; col < 3; ++col)
+				{
+					t[i * 12 + row * 4 + col] = static_cast<float>(r[row * 3 + col]);
+				}
+				t[i * 12 + row * 4 + 3] = 0.0f;
+			}
+			c[i * 4 + 0] = static_cast<float>(10000.0 + 20000.0 * uniform(rng));
+			c[i * 4 + 1] = static_cast<float>(10000.0 + 20000.0 * uniform(rng));
+			c[i * 4 + 2] = static_cast<float>(M_PI * uniform(rng));
+			c[i * 4 + 3] = 0.0f;
+		}
+	}
+	const array image_stack = to_device(images_host, ctx);
+	const array transforms = to_device(transforms_host, ctx);
+	const array ctf = to_device(ctf_host, ctx);
+	const array radial = full({h}, 1.0, numerical_type::float32, ctx);
+	array num = zeros({big, big, big / 2 + 1}, numerical_type::complex_float32, ctx);
+	array den = zeros({big, big, big / 2 + 1}, numerical_type::float32, ctx);
+	array spectra = zeros({launch, n, h}, numerical_type::complex_float32, ctx);
+	const em::gridding_window window = em::gridding_window::linear();
+	const em::ctf_optics optics(300.0, 2.7, 0.1);
+	auto part = [&](const array &a, std::size_t begin, std::size_t end) {
+		const strided_layout &l = a.get_layout();
+		std::vector<std::size_t> e;
+		std::vector<std::ptrdiff_t> st;
+		l.get_extents(e);
+		l.get_strides(st);
+		e[0] = end - begin;
+		const std::ptrdiff_t offset = l.get_offset() + static_cast<std::ptrdiff_t>(begin) * st[0];
+		return a.view(array_descriptor(strided_layout(e, st, offset), a.get_data_type()));
+	};
+	array volume;
+	auto step = [&]() {
+		for (std::size_t s = 0; s < images; s += launch)
+		{
+			const std::size_t e = std::min(s + launch, images);
+			array out = part(spectra, 0, e - s);
+			rfft2(part(image_stack, 0, e - s), ctx, &out);
+			em::insert_ctf_slices(num, den, out, part(transforms, s, e), part(ctf, s, e), radial, window, optics,
+				1.5, ctx, &num, &den);
+		}
+		volume = irfft3(num, big, ctx);
+		volume.wait();
+		den.wait();
+	};
+	step();
+	const auto t0 = std::chrono::steady_clock::now();
+	for (int k = 0; k < steps; ++k)
+	{
+		step();
+	}
+	const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+	std::cout << "{\"images\": " << images << ", \"launch\": " << launch << ", \"steps\": " << steps
+			  << ", \"ms_per_step\": " << 1e3 * seconds / steps << ", \"images_per_s\": " << images * steps / seconds
+			  << "}" << std::endl;
+	return 0;
+}
+
 } // namespace
 
 int main(int argc, char **argv)
 {
 	if (argc < 3)
 	{
-		std::cerr << "usage: plugin_driver info|run|errors <plugin> [dir]" << std::endl;
+		std::cerr << "usage: plugin_driver info|run|errors|bench <plugin> [dir | images launch steps]" << std::endl;
 		return 2;
 	}
 	const std::string mode = argv[1];
@@ -368,6 +478,10 @@ int main(int argc, char **argv)
 		if (mode == "errors")
 		{
 			return errors(argv[2]);
+		}
+		if (mode == "bench" && argc >= 6)
+		{
+			return bench(argv[2], std::stoul(argv[3]), std::stoul(argv[4]), std::stoi(argv[5]));
 		}
 	}
 	catch (const std::exception &e)

@@ -80,3 +80,14 @@ def test_errors_cross_the_boundary_as_the_reference_exceptions():
     assert e["float64"] == "unsupported_operation_error"
     assert e["host_operand"] == "unsupported_operation_error"
     assert e["new_outputs"] == "ok"
+
+
+def test_benchmark_sequence_through_the_plugin():
+    # The timed call sequence of bench.py's plugin-path leg, at a small size:
+    # rfft2 + insert_ctf_slices per launch (a partial last launch included),
+    # then irfft3, all dispatched through librexlib-hip.so.
+    rc, lines, err = driver("bench", PLUGIN, "600", "256", "1")
+    assert rc == 0, (lines, err)
+    record = lines[-1]
+    assert record["images"] == 600 and record["launch"] == 256 and record["steps"] == 1
+    assert record["images_per_s"] > 0 and record["ms_per_step"] > 0
