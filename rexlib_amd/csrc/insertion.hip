@@ -894,9 +894,10 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 		int row_start = 0;
 		int row_count = 0;
 		// Lane k describes row k of the current 64 rows (registers, not LDS,
-		// so that more wavefronts fit a CU): base position and first sample
-		// number; step and first k_x; the prepared element of (row, k_x = 0),
-		// bit 31 set for the -k copy.
+		// so that more wavefronts fit a CU): base position and the first k_x
+		// less the row's first sample number (so sample s of the batch has
+		// k_x = that + s); step; the prepared element of (row, k_x = 0), bit
+		// 31 set for the -k copy.
 		float4 row_base = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		float4 row_step = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		uint32_t row_index = 0;
@@ -959,9 +960,9 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 					b = make_float4(__shfl(row_base.x, row, 64), __shfl(row_base.y, row, 64),
 						__shfl(row_base.z, row, 64), __shfl(row_base.w, row, 64));
 					st = make_float4(__shfl(row_step.x, row, 64), __shfl(row_step.y, row, 64),
-						__shfl(row_step.z, row, 64), __shfl(row_step.w, row, 64));
+						__shfl(row_step.z, row, 64), 0.0f);
 					ix = static_cast<uint32_t>(__shfl(static_cast<int>(row_index), row, 64));
-					kx = __float_as_int(st.w) + (next + rank - __float_as_int(b.w));
+					kx = __float_as_int(b.w) + next + rank;
 					if (mine)
 					{
 						q = prepared[(ix & 0x7fffffffu) + static_cast<uint32_t>(kx)];
@@ -1179,8 +1180,8 @@ __global__ __launch_bounds__(64) void brick_kernel(insertion_params p, slice_chu
 				row_start = count_inclusive - row_count;
 				total = __builtin_amdgcn_readlane(count_inclusive, 63);
 				const int stored_row = ky < 0 ? ky + n : ky;
-				row_base = make_float4(base[0], base[1], base[2], __int_as_float(row_start));
-				row_step = make_float4(step[0], step[1], step[2], __int_as_float(begin));
+				row_base = make_float4(base[0], base[1], base[2], __int_as_float(begin - row_start));
+				row_step = make_float4(step[0], step[1], step[2], 0.0f);
 				row_index = r < total_rows ? (static_cast<uint32_t>((kept * chunk.count + js) * elements +
 					static_cast<int64_t>(stored_row) * h) | (minus ? 0x80000000u : 0u)) : 0u;
 				lds_order();
