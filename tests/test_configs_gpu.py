@@ -133,6 +133,22 @@ def test_every_brick_shape_and_share(big, tuning_table):
         tune.clear_tuning()
 
 
+@pytest.mark.parametrize("n,big", [(96, 127), (128, 193), (160, 321)])
+def test_odd_volume_extents(n, big, tuning_table):
+    # Odd N: no +-N/2 aliasing, the volume phase is the tabulated product
+    # (tabulated_phase.inl:12-37) rather than (-1)^(x+y+z), and the slices
+    # are smaller than the volume, as in padded reconstructions.
+    inputs = random_ctf_case(n, big, 200, seed=big)
+    ref = reference_insert(inputs)
+    t = [to_device(a) for a in inputs]
+    num, den = zeros(big)
+    em.insert_ctf_slices(num, den, *t[2:], em.GriddingWindow.linear(), OPTICS, 1.5,
+                         numerator_out=num, denominator_out=den)
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(num.cpu().numpy(), ref[0]) <= 1e-4
+    assert oracle.rel_l2(den.cpu().numpy(), ref[1]) <= 1e-4
+
+
 @pytest.mark.parametrize("window", [oracle.Window.nearest(), oracle.Window.kaiser_bessel(2.0, 8.0)])
 def test_every_brick_shape_with_one_and_four_taps(window, tuning_table):
     # Against the reference's float32 kernels: with tens of thousands of
