@@ -149,6 +149,40 @@ def test_odd_volume_extents(n, big, tuning_table):
     assert oracle.rel_l2(den.cpu().numpy(), ref[1]) <= 1e-4
 
 
+@pytest.mark.parametrize("variant", ["plain", "weighted"])
+def test_plain_and_weighted_insertion_into_256_cubed(variant, tuning_table):
+    # The other two insertions of the path at the headline's volume, on the
+    # tuned brick schedule, from accumulators that are not zero (in-place
+    # accumulation onto earlier sums).
+    rng = np.random.default_rng(77)
+    n = big = 256
+    count = 300
+    h = n // 2 + 1
+    slices = (rng.uniform(-1, 1, (count, n, h)) + 1j * rng.uniform(-1, 1, (count, n, h))).astype(np.complex64)
+    transforms = synthetic.transforms_of(synthetic.uniform_rotations(count, rng)).astype(np.float32)
+    radial = rng.uniform(0, 2, h).astype(np.float32)
+    num0 = (rng.uniform(-1, 1, (big, big, h)) + 1j * rng.uniform(-1, 1, (big, big, h))).astype(np.complex64)
+    den0 = rng.uniform(0, 2, (big, big, h)).astype(np.float32)
+    if variant == "plain":
+        inputs = [num0, den0, slices, transforms, radial]
+    else:
+        weights = rng.uniform(0, 2, (count, n, h)).astype(np.float32)
+        inputs = [num0, den0, slices, weights, transforms, radial]
+    if oracle.ref_available():
+        ref = oracle.ref_insert(variant, inputs, oracle.Window.linear(), dtype=np.float32, threads=THREADS)
+    else:
+        ref = oracle.insert_f64(variant, inputs, oracle.Window.linear())
+    t = [to_device(a) for a in inputs]
+    if variant == "plain":
+        num, den = em.insert_slices(*t, em.GriddingWindow.linear(), numerator_out=t[0], denominator_out=t[1])
+    else:
+        num, den = em.insert_weighted_slices(*t, em.GriddingWindow.linear(), numerator_out=t[0],
+                                             denominator_out=t[1])
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(num.cpu().numpy(), ref[0]) <= 1e-4
+    assert oracle.rel_l2(den.cpu().numpy(), ref[1]) <= 1e-4
+
+
 @pytest.mark.parametrize("window", [oracle.Window.nearest(), oracle.Window.kaiser_bessel(2.0, 8.0)])
 def test_every_brick_shape_with_one_and_four_taps(window, tuning_table):
     # Against the reference's float32 kernels: with tens of thousands of
