@@ -232,6 +232,90 @@ def learned_form(den, transforms, big, torch):
     return out
 
 
+def event_ms(torch, fn, repeats=5):
+    """Milliseconds per call from HIP events on torch's current stream (the
+    stream the library's calls run on), after one untimed call."""
+    fn()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(repeats):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / repeats
+
+
+def widened_rows(torch, em):
+    """The rows beside the hot path, each timed with HIP events against the
+    HBM roofline of its compulsory bytes (every operand read once, every
+    result written once):
+
+    * background standardization of 4,096 float32 images of 256^2 (radius 100,
+      bounds +-3, one Philox key per image);
+    * complex FFTs over the last two axes of 1,024 complex64 256^2 images, and
+      over the first axis (strided) of a 256 x 256 x 256 complex64 volume;
+    * a real FFT of 4,096 float32 rows of length 1,000 (2^3 5^3, Stockham
+      stages) and of length 1,009 (a prime: Bluestein);
+    * the quotient where(den > 1e-3, num / den, 0) over the 256^3 accumulators
+      by the library's elementwise divide, greater and where (the plugin's
+      builders), against the fused normalize kernel."""
+    from rexlib_amd import fourier, functional
+    out = {}
+
+    def entry(name, ms, nbytes):
+        out[name] = {"ms": ms, "bytes": nbytes, "gb_per_s": nbytes / (ms * 1e-3) / 1e9,
+                     "frac_of_hbm": nbytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9)}
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(17)
+    images = torch.randn((4096, 256, 256), device="cuda", generator=g)
+    keys = torch.arange(4096, dtype=torch.int64, device="cuda")
+    result = torch.empty_like(images)
+    entry("standardize_background_4096x256^2",
+          event_ms(torch, lambda: em.standardize_background(images, keys, 100.0, -3.0, 3.0, out=result)),
+          2 * images.numel() * 4 + keys.numel() * 8)
+    del images, result
+    stack = torch.randn((1024, 256, 256), dtype=torch.complex64, device="cuda", generator=g)
+    spectra = torch.empty_like(stack)
+    entry("fft_last_two_axes_1024x256^2_c64", event_ms(torch, lambda: fourier.fftn(stack, axes=(-2, -1), out=spectra)),
+          2 * stack.numel() * 8)
+    del stack, spectra
+    volume = torch.randn((256, 256, 256), dtype=torch.complex64, device="cuda", generator=g)
+    transformed = torch.empty_like(volume)
+    entry("fft_first_axis_256^3_c64", event_ms(torch, lambda: fourier.fftn(volume, axes=(0,), out=transformed)),
+          2 * volume.numel() * 8)
+    del volume, transformed
+    for length in (1000, 1009):
+        rows = torch.randn((4096, length), device="cuda", generator=g)
+        half = torch.empty((4096, length // 2 + 1), dtype=torch.complex64, device="cuda")
+        entry(f"rfft_4096_rows_of_{length}", event_ms(torch, lambda: fourier.rfftn(rows, axes=(-1,), out=half)),
+              rows.numel() * 4 + half.numel() * 8)
+        del rows, half
+    num = torch.randn((256, 256, 129), dtype=torch.complex64, device="cuda", generator=g)
+    den = torch.rand((256, 256, 129), device="cuda", generator=g)
+    den_c = den.to(torch.complex64)
+    zero = torch.zeros((), dtype=torch.complex64, device="cuda")
+    quotient = torch.empty_like(num)
+    ratio = torch.empty_like(num)
+    mask = torch.empty(den.shape, dtype=torch.bool, device="cuda")
+
+    def library_quotient():
+        functional.greater(den, 1e-3, out=mask)
+        functional.divide(num, den_c, out=ratio)
+        functional.where(mask, ratio, zero, out=quotient)
+
+    voxels = num.numel()
+    entry("quotient_by_divide_greater_where_256^3", event_ms(torch, library_quotient),
+          voxels * (8 + 8 + 4 + 8))
+    entry("quotient_by_normalize_256^3", event_ms(torch, lambda: em.normalize(num, den, 1e-3, out=quotient)),
+          voxels * (8 + 4 + 8))
+    del num, den, den_c, quotient, ratio, mask
+    torch.cuda.empty_cache()
+    log(f"widened rows: {out}")
+    return out
+
+
 def mlp_profile(torch, big=256, rows=65536, seed=11):
     """The perceptron alone: features of a random (big, big, big/2+1) weight
     volume with a coverage table of 50,000 uniform orientations, the forward
@@ -554,6 +638,9 @@ def main():
     parser.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                         help="before the workload, copy a float32 buffer of MIB MiB three times with an elementwise "
                              "kernel (known traffic, to scale hardware byte counters)")
+    parser.add_argument("--widened-rows", action="store_true",
+                        help="only the rows beside the hot path (background standardization, general FFTs, the "
+                             "elementwise quotient), timed against HBM, then exit")
     parser.add_argument("--mlp-profile", action="store_true",
                         help="only the perceptron: features of a random 256^3 weight volume, the forward pass over "
                              "every voxel three times and three training steps on 65,536 rows (the workload of the "
@@ -573,6 +660,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.mlp_profile:
         print(json.dumps(mlp_profile(torch)), flush=True)
+        return
+    if args.widened_rows:
+        print(json.dumps({"widened_rows": widened_rows(torch, em)}), flush=True)
         return
     if args.copy_calibration:
         src = torch.ones(args.copy_calibration * (1 << 18), device="cuda")
@@ -676,6 +766,7 @@ def main():
                                                           rexlib_amd),
             "learned": learned_form(den, transforms, big, torch),
         }
+        forms["widened_rows"] = widened_rows(torch, em)
         if not args.no_config_forms:
             forms["precision"] = precision_config(torch, em, rexlib_amd)
             forms["batched"] = batched_config(torch)

@@ -22,6 +22,14 @@
 // The counter of an element is its row-major position among the elements
 // sharing its key: the image axes and the batch axes the key broadcasts
 // along, in the value's axis order.
+//
+// Two kernels run these steps. The resident kernel takes float32 images
+// that are the value's last two axes, stored densely with a multiple of
+// four columns, 16-byte aligned, of at most 65,536 elements: a workgroup
+// loads its image once, 16 bytes a lane, keeps it in registers through both
+// statistics and writes the result, so the image crosses HBM once each way.
+// The general kernel takes every other layout and rank and reads its image
+// three times.
 #include "launch.hpp"
 
 #include <algorithm>
@@ -112,7 +120,8 @@ __device__ inline bool finite_double(double x)
 	return fabs(x) <= 1.7976931348623157e308;
 }
 
-// Sum of (count, sum, sum of squares) over the workgroup.
+// Sum of (count, sum, sum of squares) over a workgroup of `block` lanes.
+template <int block = threads>
 __device__ inline void block_sum(double &count, double &sum, double &squares, double *scratch)
 {
 	for (int offset = 32; offset > 0; offset /= 2)
@@ -130,15 +139,22 @@ __device__ inline void block_sum(double &count, double &sum, double &squares, do
 		scratch[3 * wave + 2] = squares;
 	}
 	__syncthreads();
-	count = 0.0;
-	sum = 0.0;
-	squares = 0.0;
-	for (int w = 0; w < threads / 64; ++w)
+	// Lane w of every wavefront takes wave w's partial sums; a butterfly
+	// over those lanes and a broadcast of lane 0 give every lane the total
+	// (the partials never sit in registers all at once).
+	constexpr int waves = block / 64;
+	count = lane < waves ? scratch[3 * lane] : 0.0;
+	sum = lane < waves ? scratch[3 * lane + 1] : 0.0;
+	squares = lane < waves ? scratch[3 * lane + 2] : 0.0;
+	for (int offset = waves / 2; offset > 0; offset /= 2)
 	{
-		count += scratch[3 * w];
-		sum += scratch[3 * w + 1];
-		squares += scratch[3 * w + 2];
+		count += __shfl_xor(count, offset, 64);
+		sum += __shfl_xor(sum, offset, 64);
+		squares += __shfl_xor(squares, offset, 64);
 	}
+	count = __shfl(count, 0, 64);
+	sum = __shfl(sum, 0, 64);
+	squares = __shfl(squares, 0, 64);
 	__syncthreads();
 }
 
@@ -230,6 +246,279 @@ __global__ __launch_bounds__(threads) void standardize_kernel(const T *value, co
 				static_cast<T>(standard_normal(image_key, static_cast<uint64_t>(counter)));
 		}
 	}
+}
+
+// The mean and 1 / deviation (population) of (count, sum, sum of squares),
+// or (0, 1) without any element (background_moments_kernel.inl:117-135).
+__device__ inline void moments(double count, double sum, double squares, double &mean, double &scale)
+{
+	if (count == 0.0)
+	{
+		mean = 0.0;
+		scale = 1.0;
+		return;
+	}
+	mean = sum / count;
+	const double variance = squares / count - mean * mean;
+	scale = variance > 0.0 ? 1.0 / sqrt(variance) : 1.0;
+}
+
+// standard_normal out of line for the resident kernel: inlined, its
+// polynomial constants are hoisted out of the image loop into registers
+// that the held image needs.
+__device__ __noinline__ float draw_float(uint64_t key, uint64_t counter)
+{
+	return static_cast<float>(standard_normal(key, counter));
+}
+
+// An empty register constraint: the compiler takes `v` as rewritten here, so
+// it converts the held floats to double afresh in the second statistics
+// instead of keeping the first statistics' doubles (twice the registers)
+// alive across the workgroup sum.
+__device__ inline void rewritten(float4 &v)
+{
+	asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+}
+
+// Geometry of the resident kernel's images: `rows` x `cols` float32
+// elements stored densely (row-major, no gaps), `cols` a multiple of 4,
+// rows * cols <= 4 * block * quads. An element is background when its
+// squared distance from (rows / 2, cols / 2) exceeds `threshold` (the
+// squared radius rounded down, clamped to INT32_MAX: for an integer d2,
+// d2 > r^2 exactly when d2 > floor(r^2)).
+struct resident_images
+{
+	int32_t rows;
+	int32_t cols;
+	int32_t threshold;
+};
+
+// One workgroup of `block` lanes per image; lane t holds the 16-byte
+// groups t, t + block, ... (`quads` of them): the first quads - staged in
+// registers, the last `staged` in LDS (each lane reads back only what it
+// wrote, so no barrier orders them). Per element the three steps
+// are those of standardize_kernel; the background and finiteness of each
+// held element is kept as one bit across the two statistics.
+template <int block, int quads, int staged>
+__global__ __launch_bounds__(block) void standardize_resident_kernel(const float *value, const uint64_t *key,
+	float *result, standardize_plan p, resident_images g)
+{
+	static_assert(4 * quads <= 64, "the background bits of a lane fit 64 bits");
+	static_assert(staged < quads, "some groups stay in registers");
+	constexpr int kept_quads = quads - staged;
+	__shared__ double scratch[3 * (block / 64)];
+	__shared__ float4 stage[staged > 0 ? staged * block : 1];
+	const int groups_per_row = g.cols / 4;
+	const int groups = g.rows * groups_per_row;
+	const int centre_row = g.rows / 2;
+	const int centre_col = g.cols / 2;
+	for (int64_t image = blockIdx.x; image < p.image_count; image += gridDim.x)
+	{
+		int64_t value_base = 0, result_base = 0, key_offset = 0, counter_base = 0;
+		int64_t rest = image;
+		for (int a = p.batch_rank - 1; a >= 0; --a)
+		{
+			const int64_t i = rest % p.batch_extent[a];
+			rest /= p.batch_extent[a];
+			value_base += i * p.batch_value_stride[a];
+			result_base += i * p.batch_result_stride[a];
+			key_offset += i * p.batch_key_stride[a];
+			counter_base += i * p.batch_counter_stride[a];
+		}
+
+		// The lane index passes through an empty register constraint once
+		// per image, so the compiler recomputes the per-group geometry in
+		// each image rather than hoisting it out of the image loop into
+		// registers the held image needs.
+		int lane = threadIdx.x;
+		asm volatile("" : "+v"(lane));
+		// Groups past the image's end load its last group again and count
+		// nowhere (no branch, so the held image stays in registers).
+		// Every group's load is issued before any is used.
+		float4 loaded[quads];
+		const float4 *in = reinterpret_cast<const float4 *>(value + value_base);
+#pragma unroll
+		for (int k = 0; k < quads; ++k)
+		{
+			loaded[k] = in[static_cast<unsigned>(min(lane + k * block, groups - 1))];
+		}
+		float4 x[kept_quads];
+#pragma unroll
+		for (int k = 0; k < quads; ++k)
+		{
+			if (k < kept_quads)
+			{
+				x[k < kept_quads ? k : 0] = loaded[k];
+			}
+			else
+			{
+				stage[(k - kept_quads) * block + lane] = loaded[k];
+			}
+		}
+		uint64_t background = 0;
+		double count = 0.0, sum = 0.0, squares = 0.0;
+#pragma unroll
+		for (int k = 0; k < quads; ++k)
+		{
+			const int q = lane + k * block;
+			const int row = q / groups_per_row;
+			const int col = 4 * (q - row * groups_per_row);
+			const int dr = row - centre_row;
+			const float lanes[4] = {loaded[k].x, loaded[k].y, loaded[k].z, loaded[k].w};
+#pragma unroll
+			for (int c = 0; c < 4; ++c)
+			{
+				const int dc = col + c - centre_col;
+				const bool valid = (q < groups) & (dr * dr + dc * dc > g.threshold) &
+					(fabsf(lanes[c]) <= 3.402823466e38f);
+				background |= static_cast<uint64_t>(valid) << (4 * k + c);
+				const double v = static_cast<double>(lanes[c]);
+				count += valid ? 1.0 : 0.0;
+				sum += valid ? v : 0.0;
+				squares += valid ? v * v : 0.0;
+			}
+		}
+		block_sum<block>(count, sum, squares, scratch);
+		double mean, scale;
+		moments(count, sum, squares, mean, scale);
+
+		count = 0.0;
+		sum = 0.0;
+		squares = 0.0;
+#pragma unroll
+		for (int k = 0; k < quads; ++k)
+		{
+			float4 held = k < kept_quads ? x[k < kept_quads ? k : 0] : stage[(k - kept_quads) * block + lane];
+			rewritten(held);
+			const float lanes[4] = {held.x, held.y, held.z, held.w};
+#pragma unroll
+			for (int c = 0; c < 4; ++c)
+			{
+				const double v = static_cast<double>(lanes[c]);
+				const double standardized = (v - mean) * scale;
+				const bool valid = (((background >> (4 * k + c)) & 1u) != 0) & (standardized >= p.lower_bound) &
+					(standardized <= p.upper_bound);
+				count += valid ? 1.0 : 0.0;
+				sum += valid ? v : 0.0;
+				squares += valid ? v * v : 0.0;
+			}
+		}
+		block_sum<block>(count, sum, squares, scratch);
+		moments(count, sum, squares, mean, scale);
+
+		const float mean_t = static_cast<float>(mean);
+		const float scale_t = static_cast<float>(scale);
+		const float lower_t = static_cast<float>(p.lower_bound);
+		const float upper_t = static_cast<float>(p.upper_bound);
+		// Standardized values go out as held; the elements to be replaced
+		// (few: outliers and non-finite values) are marked and then
+		// overwritten one by one by the same lane, after the held image is
+		// dead.
+		uint64_t drawn = 0;
+		float4 *out = reinterpret_cast<float4 *>(result + result_base);
+		asm volatile("" : "+v"(lane));
+#pragma unroll
+		for (int k = 0; k < quads; ++k)
+		{
+			const int q = lane + k * block;
+			const float4 held = k < kept_quads ? x[k < kept_quads ? k : 0] : stage[(k - kept_quads) * block + lane];
+			float lanes[4] = {held.x, held.y, held.z, held.w};
+#pragma unroll
+			for (int c = 0; c < 4; ++c)
+			{
+				float standardized;
+				{
+#pragma clang fp contract(off)
+					standardized = (lanes[c] - mean_t) * scale_t;
+				}
+				const bool kept = (standardized >= lower_t) & (standardized <= upper_t) &
+					(fabsf(standardized) <= 3.402823466e38f);
+				drawn |= static_cast<uint64_t>(!kept & (q < groups)) << (4 * k + c);
+				lanes[c] = standardized;
+			}
+			if (q < groups)
+			{
+				out[static_cast<unsigned>(q)] = make_float4(lanes[0], lanes[1], lanes[2], lanes[3]);
+			}
+		}
+		if (drawn != 0)
+		{
+			const uint64_t image_key = key[key_offset];
+			do
+			{
+				const int bit = __builtin_ctzll(drawn);
+				drawn &= drawn - 1;
+				const int e = 4 * (lane + (bit / 4) * block) + bit % 4;
+				result[result_base + e] = draw_float(image_key, static_cast<uint64_t>(counter_base + e));
+			} while (drawn != 0);
+		}
+	}
+}
+
+template <int block, int quads, int staged = 0>
+void launch_resident(unsigned grid, hipStream_t s, const float *value, const uint64_t *key, float *result,
+	const standardize_plan &p, const resident_images &g)
+{
+	standardize_resident_kernel<block, quads, staged><<<grid, block, 0, s>>>(value, key, result, p, g);
+}
+
+// Launches the resident kernel when the operands fit it; false otherwise.
+bool try_resident(hipStream_t s, const rexlib_hip_array *value, const rexlib_hip_array *result, const uint64_t *key,
+	uint32_t image_axes, const standardize_plan &p, unsigned grid)
+{
+	const int rank = value->rank;
+	if (value->dtype != REXLIB_HIP_FLOAT32 || p.image_rank != 2 || rank < 2 || image_axes != (3u << (rank - 2)))
+	{
+		return false;
+	}
+	const int64_t rows = p.image_extent[0];
+	const int64_t cols = p.image_extent[1];
+	if (cols % 4 != 0 || rows * cols > 65536 || p.image_value_stride[1] != 1 || p.image_result_stride[1] != 1 ||
+		p.image_counter_stride[0] != cols || p.image_counter_stride[1] != 1 || p.image_value_stride[0] != cols ||
+		p.image_result_stride[0] != cols || reinterpret_cast<uintptr_t>(value->data) % 16 != 0 ||
+		reinterpret_cast<uintptr_t>(result->data) % 16 != 0)
+	{
+		return false;
+	}
+	for (int k = 0; k < p.batch_rank; ++k)
+	{
+		if (p.batch_value_stride[k] % 4 != 0 || p.batch_result_stride[k] % 4 != 0)
+		{
+			return false;
+		}
+	}
+	resident_images g{};
+	g.rows = static_cast<int32_t>(rows);
+	g.cols = static_cast<int32_t>(cols);
+	g.threshold = static_cast<int32_t>(std::min(std::floor(p.squared_radius), static_cast<double>(INT32_MAX)));
+	const float *v = static_cast<const float *>(value->data);
+	float *r = static_cast<float *>(result->data);
+	const int64_t elements = rows * cols;
+	if (elements <= 256 * 4 * 1)
+	{
+		launch_resident<256, 1>(grid, s, v, key, r, p, g);
+	}
+	else if (elements <= 256 * 4 * 2)
+	{
+		launch_resident<256, 2>(grid, s, v, key, r, p, g);
+	}
+	else if (elements <= 256 * 4 * 4)
+	{
+		launch_resident<256, 4>(grid, s, v, key, r, p, g);
+	}
+	else if (elements <= 1024 * 4 * 4)
+	{
+		launch_resident<1024, 4>(grid, s, v, key, r, p, g);
+	}
+	else if (elements <= 1024 * 4 * 8)
+	{
+		launch_resident<1024, 8, 0>(grid, s, v, key, r, p, g);
+	}
+	else
+	{
+		launch_resident<1024, 16, 9>(grid, s, v, key, r, p, g);
+	}
+	return true;
 }
 
 } // anonymous namespace
@@ -347,7 +636,10 @@ int rexlib_hip_standardize_background(const rexlib_hip_array *value, const rexli
 		const hipStream_t s = static_cast<hipStream_t>(stream);
 		const unsigned grid = static_cast<unsigned>(std::min<int64_t>(p.image_count, 1 << 20));
 		const uint64_t *k = static_cast<const uint64_t *>(key->data);
-		if (value->dtype == REXLIB_HIP_FLOAT32)
+		if (try_resident(s, value, result, k, image_axes, p, grid))
+		{
+		}
+		else if (value->dtype == REXLIB_HIP_FLOAT32)
 		{
 			standardize_kernel<float><<<grid, threads, 0, s>>>(static_cast<const float *>(value->data), k,
 				static_cast<float *>(result->data), p);

@@ -102,3 +102,44 @@ def test_invalid_operands_are_refused():
         em.standardize_background(x, torch.zeros(3, dtype=torch.int64, device="cuda"), 1.0, -3.0, 3.0)
     with pytest.raises(em.UnsupportedOperationError):
         em.standardize_background(x.half(), k, 1.0, -3.0, 3.0)
+
+
+# Densely stored float32 images with a multiple of four columns run the
+# register-resident kernel (one read and one write of each image); the
+# shapes cover each of its register/LDS splits, an image that fills its
+# workgroup's lanes in part (100 x 100, 20 x 8), a batch with a gap between
+# images, a broadcast key and a radius beyond every element.
+@pytest.mark.parametrize("shape,radius", [((3, 256, 256), 100.0), ((4, 128, 128), 50.0), ((5, 100, 100), 30.5),
+                                          ((6, 64, 64), 20.0), ((7, 20, 8), 3.0), ((3, 176, 176), 70.0),
+                                          ((2, 32, 32), 1000.0)])
+def test_resident_kernel_shapes(shape, radius):
+    print("standardize resident", shape, radius, flush=True)
+    x = images(shape, seed=shape[1] + 3)
+    keys = np.arange(7, 7 + shape[0], dtype=np.uint64)
+    want = oracle.standardize_background(x, keys, radius, -3.0, 3.0)
+    got = em.standardize_background(torch.from_numpy(x).cuda(), torch.from_numpy(keys.view(np.int64)).cuda(), radius,
+                                    -3.0, 3.0).cpu().numpy()
+    compare(got, want)
+
+
+def test_resident_kernel_batch_with_gaps_and_a_shared_key():
+    x = images((6, 64, 64), seed=21)
+    strided = torch.from_numpy(x).cuda()[::2]  # images 0, 2, 4: a gap of one image between them
+    want = oracle.standardize_background(x[::2], np.array(5, dtype=np.uint64), 12.0, -2.0, 2.0)
+    got = em.standardize_background(strided, torch.tensor(5, dtype=torch.int64).cuda(), 12.0, -2.0, 2.0)
+    compare(got.cpu().numpy(), want)
+
+
+def test_resident_and_general_kernels_agree_on_an_offset_view():
+    # An image view that starts one element into its buffer is not 16-byte
+    # aligned and takes the general kernel; the aligned copy takes the
+    # resident one. Both follow the same steps.
+    x = images((3, 48, 48), seed=23)
+    buffer = torch.zeros(x.size + 1, device="cuda")
+    buffer[1:] = torch.from_numpy(x.reshape(-1)).cuda()
+    offset = buffer[1:].view(3, 48, 48)
+    keys = torch.arange(3, dtype=torch.int64).cuda()
+    general = em.standardize_background(offset, keys, 15.0, -3.0, 3.0).cpu().numpy()
+    resident = em.standardize_background(torch.from_numpy(x).cuda(), keys, 15.0, -3.0, 3.0).cpu().numpy()
+    compare(resident, general)
+    compare(resident, oracle.standardize_background(x, np.arange(3, dtype=np.uint64), 15.0, -3.0, 3.0))
