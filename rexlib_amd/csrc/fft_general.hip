@@ -4,7 +4,9 @@
 // reference's CPU backend (src/backends/cpu/kernels/pocketfft_fourier_transform.cpp:74-163,
 // src/ops/fourier/fourier_normalization.cpp:56-71).
 //
-// The operand is gathered into a contiguous complex buffer; each axis is
+// A complex transform between dense operands over axes of 2-3-5 lengths
+// runs its axis passes from the input straight into the output. Otherwise
+// the operand is gathered into a contiguous complex buffer; each axis is
 // then a batch of lines (outer, line, inner) transformed in place by the
 // Stockham kernels (fft.hip) when its length factors into 2, 3 and 5, and by
 // Bluestein's chirp-z algorithm otherwise: x_k w_k, zero-padded to a power
@@ -420,6 +422,69 @@ void check_fft_array(const rexlib_hip_array *a, const char *what)
 }
 
 // The whole general transform. kind: 0 c2c, 1 r2c, 2 c2r.
+// True when the array's elements are stored row-major with no gaps (axes of
+// extent 1 may carry any stride).
+bool dense(const rexlib_hip_array &a)
+{
+	int64_t step = 1;
+	for (int d = a.rank - 1; d >= 0; --d)
+	{
+		if (a.extents[d] != 1 && a.strides[d] != step)
+		{
+			return false;
+		}
+		step *= a.extents[d];
+	}
+	return true;
+}
+
+// A complex transform between dense operands (the same buffer, or buffers
+// that do not overlap) over axes whose lengths factor into 2, 3 and 5: the
+// first axis pass reads the input and writes the output, the later passes
+// run in place on the output, the last one applying the normalisation. No
+// gather, scatter or workspace. Returns false when the operands do not fit.
+bool direct_complex_transform(const rexlib_hip_array *in, const rexlib_hip_array *out, const std::vector<int> &axes,
+	const std::vector<int64_t> &full, float sign, float scale, int64_t count, hipStream_t stream)
+{
+	if (!dense(*in) || !dense(*out))
+	{
+		return false;
+	}
+	const char *a = static_cast<const char *>(in->data);
+	const char *b = static_cast<const char *>(out->data);
+	const int64_t bytes = count * static_cast<int64_t>(sizeof(float2));
+	if (a != b && a < b + bytes && b < a + bytes)
+	{
+		return false;
+	}
+	std::vector<int> live;
+	for (const int axis : axes)
+	{
+		const int64_t n = full[axis];
+		if (n > 1)
+		{
+			if (!smooth(n) || n > longest_direct)
+			{
+				return false;
+			}
+			live.push_back(axis);
+		}
+	}
+	if (live.empty())
+	{
+		return false;
+	}
+	const float2 *src = static_cast<const float2 *>(in->data);
+	float2 *dst = static_cast<float2 *>(out->data);
+	for (std::size_t i = 0; i < live.size(); ++i)
+	{
+		const line_set lines = axis_lines(full, live[i]);
+		launch_fft_lines(i == 0 ? src : dst, lines, dst, lines, full[live[i]], sign,
+			i + 1 == live.size() ? scale : 1.0f, stream);
+	}
+	return true;
+}
+
 void general_transform(int kind, const rexlib_hip_array *in, const rexlib_hip_array *out, const int32_t *axes_in,
 	int32_t axis_count, bool inverse, int normalization, hipStream_t stream)
 {
@@ -472,6 +537,11 @@ void general_transform(int kind, const rexlib_hip_array *in, const rexlib_hip_ar
 	}
 	const float scale = static_cast<float>(normalisation_factor(normalization, inverse, points));
 	const float sign = inverse ? 1.0f : -1.0f;
+	if (kind == 0 && direct_complex_transform(in, out, axes, full, sign, scale, count, stream))
+	{
+		REXLIB_HIP_CHECK(hipGetLastError());
+		return;
+	}
 
 	// Workspace: the contiguous buffer, the c2r half lines, and Bluestein scratch.
 	std::vector<int64_t> work_extents = full;

@@ -71,6 +71,24 @@ def test_strided_operands_and_aliasing():
     close(y.cpu().numpy(), want)
 
 
+# Dense operands over axes of 2-3-5 lengths take the direct path (axis passes
+# from the input straight into the output, the normalisation in the last);
+# the cases cover an image stack over its last two axes, an axis of extent 1
+# among the transformed ones, and a transform in place.
+@pytest.mark.parametrize("shape,axes", [((16, 256, 256), (-2, -1)), ((1, 8, 12), (0, 1)), ((3, 20, 45), (2, 0)),
+                                         ((256, 4, 6), (0,))])
+@pytest.mark.parametrize("norm", NORMS)
+def test_direct_complex_transforms(shape, axes, norm):
+    print("direct fft", shape, axes, norm, flush=True)
+    rng = np.random.default_rng(sum(shape) + len(norm))
+    x = crandn(rng, shape)
+    t = torch.from_numpy(x).cuda()
+    close(fourier.fftn(t, axes, norm).cpu().numpy(), np.fft.fftn(x.astype(np.complex128), axes=axes, norm=norm))
+    close(fourier.ifftn(t, axes, norm).cpu().numpy(), np.fft.ifftn(x.astype(np.complex128), axes=axes, norm=norm))
+    fourier.fftn(t, axes, norm, out=t)
+    close(t.cpu().numpy(), np.fft.fftn(x.astype(np.complex128), axes=axes, norm=norm))
+
+
 def test_irfft_ignores_the_imaginary_parts_of_self_conjugate_coefficients():
     rng = np.random.default_rng(7)
     spec = crandn(rng, (8, 5))  # n = 8 along the last axis
