@@ -34,6 +34,15 @@ struct walk
 
 __device__ inline void offsets_of(const walk &w, int64_t i, int operands, int64_t *out)
 {
+	if (w.rank == 1)
+	{
+		// Dense operands (and scalars) collapse to one axis: no division.
+		for (int k = 0; k < operands; ++k)
+		{
+			out[k] = i * w.strides[k][0];
+		}
+		return;
+	}
 	for (int k = 0; k < operands; ++k)
 	{
 		out[k] = 0;
@@ -159,12 +168,55 @@ walk make_walk(const char *what, const rexlib_hip_array &result, const rexlib_hi
 			w.strides[k + 1][a] = e == 1 ? 0 : x.strides[a - lead];
 		}
 	}
-	return w;
+	// Merge axis a into axis a + 1 wherever every operand steps across the
+	// pair as across one axis (a broadcast operand steps 0 across both), and
+	// drop axes of extent 1: dense operands of one shape become one axis.
+	int rank = 0;
+	walk c = w;
+	for (int a = 0; a < w.rank; ++a)
+	{
+		if (w.extents[a] == 1 && w.rank > 1)
+		{
+			continue;
+		}
+		bool merge = rank > 0;
+		for (int k = 0; k <= count && merge; ++k)
+		{
+			merge = c.strides[k][rank - 1] == w.strides[k][a] * w.extents[a];
+		}
+		if (merge)
+		{
+			c.extents[rank - 1] *= w.extents[a];
+			for (int k = 0; k <= count; ++k)
+			{
+				c.strides[k][rank - 1] = w.strides[k][a];
+			}
+			continue;
+		}
+		c.extents[rank] = w.extents[a];
+		for (int k = 0; k <= count; ++k)
+		{
+			c.strides[k][rank] = w.strides[k][a];
+		}
+		++rank;
+	}
+	if (rank == 0)
+	{
+		// Every axis of extent 1: one element.
+		c.extents[0] = 1;
+		for (int k = 0; k <= count; ++k)
+		{
+			c.strides[k][0] = 0;
+		}
+		rank = 1;
+	}
+	c.rank = rank;
+	return c;
 }
 
 void check_operand(const rexlib_hip_array *a, const char *what)
 {
-	if (a == nullptr || (a->data == nullptr && a->rank > 0))
+	if (a == nullptr)
 	{
 		fail(REXLIB_HIP_INVALID_ARGUMENT, std::string(what) + ": missing operand");
 	}
@@ -172,12 +224,19 @@ void check_operand(const rexlib_hip_array *a, const char *what)
 	{
 		fail(REXLIB_HIP_INVALID_ARGUMENT, std::string(what) + ": rank beyond " + std::to_string(max_rank));
 	}
+	bool empty = false;
 	for (int i = 0; i < a->rank; ++i)
 	{
 		if (a->extents[i] < 0)
 		{
 			fail(REXLIB_HIP_INVALID_ARGUMENT, std::string(what) + ": negative extent");
 		}
+		empty = empty || a->extents[i] == 0;
+	}
+	// An operand with no elements may have no storage.
+	if (a->data == nullptr && !empty)
+	{
+		fail(REXLIB_HIP_INVALID_ARGUMENT, std::string(what) + ": missing operand");
 	}
 }
 

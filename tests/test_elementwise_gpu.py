@@ -73,3 +73,28 @@ def test_quotient_recipe_equals_normalize_and_the_oracle():
     ref = oracle.normalize(num.astype(np.complex128), den.astype(np.float64), 1e-3)
     assert np.max(np.abs(q.cpu().numpy() - ref)) <= 1e-6 * np.max(np.abs(ref))
     assert np.array_equal(q.cpu().numpy(), em.normalize(n_dev, d_dev, 1e-3).cpu().numpy())
+
+
+# The walk merges neighbouring axes that every operand steps across as one
+# (dense operands become a single axis) and drops axes of extent 1; the
+# cases mix merged and unmerged axes, a broadcast along outer axes only, a
+# transposed view, extent-1 axes between others, a 0-d result and an empty
+# one.
+@pytest.mark.parametrize("a_shape,b_shape,transpose", [((3, 4, 5, 6), (5, 6), False), ((3, 1, 5, 6), (3, 1, 1, 6), False),
+                                                       ((7, 8), (7, 1), True), ((2, 3, 4), (1, 3, 1), False),
+                                                       ((), (), False), ((4, 0, 3), (3,), False),
+                                                       ((1, 1, 9), (1, 9), False)])
+def test_divide_greater_where_over_merged_axes(a_shape, b_shape, transpose):
+    rng = np.random.default_rng(len(a_shape) + len(b_shape))
+    a = np.asarray(rng.normal(size=a_shape), dtype=np.float32)
+    b = np.asarray(rng.uniform(0.5, 2.0, size=b_shape), dtype=np.float32)
+    ta, tb = (torch.from_numpy(np.array(v)).cuda() for v in (a, b))  # np.array keeps a 0-d shape
+    if transpose:
+        ta, a = ta.t(), a.T
+        tb, b = to_device(np.ascontiguousarray(b.T)), b.T
+    got = F.divide(ta, tb).cpu().numpy()
+    np.testing.assert_allclose(got, a.astype(np.float64) / b, rtol=2e-6, atol=0)
+    mask = F.greater(ta, tb)
+    assert np.array_equal(mask.cpu().numpy(), a > b)
+    picked = F.where(mask, ta, tb).cpu().numpy()
+    assert np.array_equal(picked, np.where(a > b, a, np.broadcast_to(b, np.broadcast(a, b).shape)))
