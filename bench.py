@@ -259,7 +259,8 @@ def widened_rows(torch, em):
       stages) and of length 1,009 (a prime: Bluestein);
     * the quotient where(den > 1e-3, num / den, 0) over the 256^3 accumulators
       by the library's elementwise divide, greater and where (the plugin's
-      builders), against the fused normalize kernel."""
+      builders), against the fused normalize kernel;
+    * the insertion builders with the other windows and weights (images/s)."""
     from rexlib_amd import fourier, functional
     out = {}
 
@@ -311,6 +312,41 @@ def widened_rows(torch, em):
     entry("quotient_by_normalize_256^3", event_ms(torch, lambda: em.normalize(num, den, 1e-3, out=quotient)),
           voxels * (8 + 4 + 8))
     del num, den, den_c, quotient, ratio, mask
+
+    # The insertion builders beside the fused CTF path: 4,096 slices of
+    # 256 x 129 spectra into 256^3 accumulators per call, uniform views,
+    # by insert_slices (trilinear), insert_weighted_slices (trilinear),
+    # insert_ctf_slices with the nearest window and with a Kaiser-Bessel
+    # window of half-width 2 (shape 7.4). Images/s per call.
+    from rexlib_amd import synthetic
+    count, n = 4096, 256
+    rng = np.random.default_rng(29)
+    transforms = torch.from_numpy(synthetic.transforms_of(synthetic.uniform_rotations(count, rng)).astype(
+        np.float32)).cuda()
+    ctf = torch.from_numpy(synthetic.ctf_parameters(count, rng).astype(np.float32)).cuda()
+    slices = torch.randn((count, n, n // 2 + 1), dtype=torch.complex64, device="cuda", generator=g)
+    weights = torch.rand((count, n, n // 2 + 1), device="cuda", generator=g)
+    radial = torch.ones(n // 2 + 1, device="cuda")
+    num = torch.zeros((n, n, n // 2 + 1), dtype=torch.complex64, device="cuda")
+    den = torch.zeros((n, n, n // 2 + 1), device="cuda")
+    optics = em.CtfOptics(300.0, 2.7, 0.1)
+    linear = em.GriddingWindow.linear()
+    calls = {
+        "insert_slices_linear": lambda: em.insert_slices(num, den, slices, transforms, radial, linear,
+                                                         numerator_out=num, denominator_out=den),
+        "insert_weighted_slices_linear": lambda: em.insert_weighted_slices(
+            num, den, slices, weights, transforms, radial, linear, numerator_out=num, denominator_out=den),
+        "insert_ctf_slices_nearest": lambda: em.insert_ctf_slices(
+            num, den, slices, transforms, ctf, radial, em.GriddingWindow.nearest(), optics, 1.0,
+            numerator_out=num, denominator_out=den),
+        "insert_ctf_slices_kaiser_bessel_2": lambda: em.insert_ctf_slices(
+            num, den, slices, transforms, ctf, radial, em.GriddingWindow.kaiser_bessel(2.0, 7.4), optics, 1.0,
+            numerator_out=num, denominator_out=den),
+    }
+    for name, call in calls.items():
+        ms = event_ms(torch, call, repeats=3)
+        out[name] = {"ms": ms, "images_per_s": count / (ms * 1e-3)}
+    del slices, weights, num, den
     torch.cuda.empty_cache()
     log(f"widened rows: {out}")
     return out
