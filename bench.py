@@ -698,6 +698,7 @@ def main():
         print(json.dumps(mlp_profile(torch)), flush=True)
         return
     if args.widened_rows:
+        tune.apply_all()
         print(json.dumps({"widened_rows": widened_rows(torch, em)}), flush=True)
         return
     if args.copy_calibration:
