@@ -127,19 +127,21 @@ __device__ inline float2 chirp(int64_t k, int64_t n, float sign)
 	return make_float2(static_cast<float>(c), static_cast<float>(sign * s));
 }
 
-// Bluestein step 1: a[l][k] = x[l][k] exp(sign i pi k^2 / n) for k < n, 0 up to m.
-__global__ void chirp_in_kernel(const float2 *x, line_set lines, int64_t n, int64_t m, float sign, float2 *a)
+// Bluestein step 1: a[l][k] = x[l][k] w[k] for k < n, 0 up to m, with
+// w[k] = exp(sign i pi k^2 / n) from the cached table and m = 2^m_shift.
+__global__ void chirp_in_kernel(const float2 *x, line_set lines, int64_t n, int m_shift, const float2 *w, float2 *a)
 {
+	const int64_t m = int64_t(1) << m_shift;
 	const int64_t total = lines.count * m;
 	for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
 		i += static_cast<int64_t>(gridDim.x) * blockDim.x)
 	{
-		const int64_t l = i / m;
-		const int64_t k = i - l * m;
+		const int64_t l = i >> m_shift;
+		const int64_t k = i & (m - 1);
 		float2 v = make_float2(0.0f, 0.0f);
 		if (k < n)
 		{
-			v = cmul(x[line_offset(lines, l) + k * lines.s_elem], chirp(k, n, sign));
+			v = cmul(x[line_offset(lines, l) + k * lines.s_elem], w[k]);
 		}
 		a[i] = v;
 	}
@@ -156,7 +158,7 @@ __global__ void chirp_multiply_kernel(float2 *a, const float2 *b, int64_t m, int
 }
 
 // Bluestein step 5: x[l][k] = scale exp(sign i pi k^2 / n) a[l][k], k < n.
-__global__ void chirp_out_kernel(const float2 *a, int64_t n, int64_t m, float sign, float scale, float2 *x,
+__global__ void chirp_out_kernel(const float2 *a, int64_t n, int64_t m, const float2 *w, float scale, float2 *x,
 	line_set lines)
 {
 	const int64_t total = lines.count * n;
@@ -165,7 +167,7 @@ __global__ void chirp_out_kernel(const float2 *a, int64_t n, int64_t m, float si
 	{
 		const int64_t l = i / n;
 		const int64_t k = i - l * n;
-		x[line_offset(lines, l) + k * lines.s_elem] = cscale(cmul(a[l * m + k], chirp(k, n, sign)), scale);
+		x[line_offset(lines, l) + k * lines.s_elem] = cscale(cmul(a[l * m + k], w[k]), scale);
 	}
 }
 
@@ -249,7 +251,18 @@ int64_t bluestein_length(int64_t n)
 	return m;
 }
 
-// B = FFT_m(conjugate chirp), per device, length and sign; built once.
+// The chirp w[k] = exp(sign i pi k^2 / n), k < n.
+__global__ void chirp_values_kernel(int64_t n, float sign, float2 *w)
+{
+	for (int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; k < n;
+		k += static_cast<int64_t>(gridDim.x) * blockDim.x)
+	{
+		w[k] = chirp(k, n, sign);
+	}
+}
+
+// B = FFT_m(conjugate chirp) in the first m elements, then the chirp w
+// itself (n elements), per device, length and sign; built once.
 const float2 *chirp_spectrum(int64_t n, int64_t m, float sign, hipStream_t stream)
 {
 	static std::mutex mutex;
@@ -264,8 +277,9 @@ const float2 *chirp_spectrum(int64_t n, int64_t m, float sign, hipStream_t strea
 		return found->second;
 	}
 	float2 *b = nullptr;
-	REXLIB_HIP_CHECK(hipMalloc(&b, sizeof(float2) * static_cast<std::size_t>(m)));
+	REXLIB_HIP_CHECK(hipMalloc(&b, sizeof(float2) * static_cast<std::size_t>(m + n)));
 	chirp_table_kernel<<<blocks_for(m), 256, 0, stream>>>(n, m, sign, b);
+	chirp_values_kernel<<<blocks_for(n), 256, 0, stream>>>(n, sign, b + m);
 	const line_set one{1, 1, 1, 0, 0, 0, 1};
 	launch_fft_lines(b, one, b, one, m, -1.0f, 1.0f, stream);
 	REXLIB_HIP_CHECK(hipStreamSynchronize(stream));
@@ -314,12 +328,17 @@ void transform_axis(float2 *buffer, const std::vector<int64_t> &extents, int axi
 	}
 	const float2 *b = chirp_spectrum(n, m, sign, stream);
 	const int64_t total = lines.count * m;
-	chirp_in_kernel<<<blocks_for(total), 256, 0, stream>>>(buffer, lines, n, m, sign, scratch);
+	int m_shift = 0;
+	while ((int64_t(1) << m_shift) < m)
+	{
+		++m_shift;
+	}
+	chirp_in_kernel<<<blocks_for(total), 256, 0, stream>>>(buffer, lines, n, m_shift, b + m, scratch);
 	const line_set padded{lines.count, 1, 1, m, 0, 0, 1};
 	launch_fft_lines(scratch, padded, scratch, padded, m, -1.0f, 1.0f, stream);
 	chirp_multiply_kernel<<<blocks_for(total), 256, 0, stream>>>(scratch, b, m, total);
 	launch_fft_lines(scratch, padded, scratch, padded, m, 1.0f, 1.0f / static_cast<float>(m), stream);
-	chirp_out_kernel<<<blocks_for(lines.count * n), 256, 0, stream>>>(scratch, n, m, sign, 1.0f, buffer, lines);
+	chirp_out_kernel<<<blocks_for(lines.count * n), 256, 0, stream>>>(scratch, n, m, b + m, 1.0f, buffer, lines);
 	REXLIB_HIP_CHECK(hipGetLastError());
 }
 
