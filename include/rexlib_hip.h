@@ -110,6 +110,12 @@ int rexlib_hip_set_insertion_tuning(int32_t extent, int32_t extent_x, int32_t ex
 	int32_t share);
 int rexlib_hip_get_insertion_tuning(int32_t extent, int32_t *extent_x, int32_t *extent_y, int32_t *extent_z,
 	int32_t *share);
+/* The same for one window family: `taps` per axis of the brick layout, 1
+ * (nearest), 2 (trilinear) or 4 (every Kaiser-Bessel window). A call with
+ * that window reads this entry before the extent's. No reference
+ * counterpart; set by the autotuner (rexlib_amd/tune.py). */
+int rexlib_hip_set_insertion_tuning_for_window(int32_t extent, int32_t taps, int32_t extent_x, int32_t extent_y,
+	int32_t extent_z, int32_t share);
 /* Drops every entry (the defaults apply again). */
 void rexlib_hip_clear_insertion_tuning(void);
 

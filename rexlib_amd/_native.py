@@ -58,6 +58,7 @@ EXPORTS = [
     "rexlib_hip_version",
     "rexlib_hip_device_count",
     "rexlib_hip_set_insertion_tuning",
+    "rexlib_hip_set_insertion_tuning_for_window",
     "rexlib_hip_get_insertion_tuning",
     "rexlib_hip_clear_insertion_tuning",
     "rexlib_hip_insert_slices",
@@ -114,6 +115,8 @@ def library() -> ctypes.CDLL:
     lib.rexlib_hip_device_count.argtypes = []
     lib.rexlib_hip_set_insertion_tuning.argtypes = [ctypes.c_int32] * 5
     lib.rexlib_hip_set_insertion_tuning.restype = ctypes.c_int
+    lib.rexlib_hip_set_insertion_tuning_for_window.argtypes = [ctypes.c_int32] * 6
+    lib.rexlib_hip_set_insertion_tuning_for_window.restype = ctypes.c_int
     lib.rexlib_hip_get_insertion_tuning.argtypes = [ctypes.c_int32] + [P(ctypes.c_int32)] * 4
     lib.rexlib_hip_get_insertion_tuning.restype = ctypes.c_int
     lib.rexlib_hip_clear_insertion_tuning.argtypes = []

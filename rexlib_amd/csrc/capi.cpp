@@ -493,7 +493,13 @@ void rexlib_hip_version(int32_t *major, int32_t *minor, int32_t *patch)
 int rexlib_hip_set_insertion_tuning(int32_t extent, int32_t extent_x, int32_t extent_y, int32_t extent_z,
 	int32_t share)
 {
-	return guarded([&] { rexlib_hip::set_insertion_tuning(extent, extent_x, extent_y, extent_z, share); });
+	return guarded([&] { rexlib_hip::set_insertion_tuning(extent, 0, extent_x, extent_y, extent_z, share); });
+}
+
+int rexlib_hip_set_insertion_tuning_for_window(int32_t extent, int32_t taps, int32_t extent_x, int32_t extent_y,
+	int32_t extent_z, int32_t share)
+{
+	return guarded([&] { rexlib_hip::set_insertion_tuning(extent, taps, extent_x, extent_y, extent_z, share); });
 }
 
 int rexlib_hip_get_insertion_tuning(int32_t extent, int32_t *extent_x, int32_t *extent_y, int32_t *extent_z,

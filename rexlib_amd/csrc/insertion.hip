@@ -1283,10 +1283,21 @@ bool valid_brick_shape(int ex, int ey, int ez)
 		(ex == 8 && ey == 8 && (ez == 8 || ez == 6 || ez == 4)) || (ex == 8 && ey == 6 && ez == 6);
 }
 
-brick_tuning tuning_for(int extent)
+// Table keys: the extent (every window), or the extent plus taps_key times
+// the taps per axis of the brick layout (1, 2 or 4) for one window family.
+constexpr int taps_key = 1 << 20;
+
+// The tuning of an extent for a window of `taps` per axis (0: any window):
+// the window's own entry, else the extent's, else the entry for every
+// extent, else the built-in default.
+brick_tuning tuning_for(int extent, int taps = 0)
 {
 	std::lock_guard<std::mutex> lock(tuning_mutex);
-	auto found = tuning_table.find(extent);
+	auto found = taps > 0 ? tuning_table.find(extent + taps * taps_key) : tuning_table.end();
+	if (found == tuning_table.end())
+	{
+		found = tuning_table.find(extent);
+	}
 	if (found == tuning_table.end())
 	{
 		found = tuning_table.find(0);
@@ -1607,11 +1618,17 @@ const int4 *brick_tasks(int big, int64_t slices, const brick_tuning &tuning, int
 
 } // anonymous namespace
 
-void set_insertion_tuning(int extent, int extent_x, int extent_y, int extent_z, int share)
+void set_insertion_tuning(int extent, int taps, int extent_x, int extent_y, int extent_z, int share)
 {
-	if (extent < 0)
+	if (extent < 0 || extent >= taps_key)
 	{
-		fail(REXLIB_HIP_INVALID_ARGUMENT, "insertion tuning: the extent must be 0 (every extent) or positive");
+		fail(REXLIB_HIP_INVALID_ARGUMENT, "insertion tuning: the extent must be 0 (every extent) or positive, "
+			"below 2^20");
+	}
+	if (taps != 0 && taps != 1 && taps != 2 && taps != 4)
+	{
+		fail(REXLIB_HIP_INVALID_ARGUMENT, "insertion tuning: taps per axis are 0 (every window), 1 (nearest), "
+			"2 (trilinear) or 4 (Kaiser-Bessel)");
 	}
 	if (!valid_brick_shape(extent_x, extent_y, extent_z))
 	{
@@ -1623,7 +1640,7 @@ void set_insertion_tuning(int extent, int extent_x, int extent_y, int extent_z, 
 		fail(REXLIB_HIP_INVALID_ARGUMENT, "insertion tuning: plane hits per task lie in [16, 2^20]");
 	}
 	std::lock_guard<std::mutex> lock(tuning_mutex);
-	tuning_table[extent] = brick_tuning{extent_x, extent_y, extent_z, share};
+	tuning_table[extent + taps * taps_key] = brick_tuning{extent_x, extent_y, extent_z, share};
 }
 
 void get_insertion_tuning(int extent, int *extent_x, int *extent_y, int *extent_z, int *share)
@@ -1718,9 +1735,9 @@ void run_insertion(insertion_params p, int schedule, const float2 *num_in, const
 	const int64_t per_slice = static_cast<int64_t>(p.n) * p.h * static_cast<int64_t>(sizeof(prepared_sample)) *
 		p.map.kept_count;
 	const int64_t limit = int64_t(4) << 30;
-	const brick_tuning tuning = tuning_for(p.big);
 	// Taps per axis of the brick layout: 1, 2, or 4 for every Kaiser-Bessel window.
 	const int layout_taps = p.profile == REXLIB_HIP_BOX ? 1 : (p.profile == REXLIB_HIP_TRIANGLE ? 2 : 4);
+	const brick_tuning tuning = tuning_for(p.big, layout_taps);
 	int64_t chunk_count = 0;
 	if (schedule != schedule_scatter && p.taps <= 4 && p.map.kept_count <= 65535)
 	{

@@ -109,7 +109,7 @@ void insert_ctf_slices_within(const rexlib_hip_array *numerator, const rexlib_hi
 
 // Brick extents along x, y and z and the plane hits per task of the brick
 // schedule, per volume extent (0: every extent without an entry).
-void set_insertion_tuning(int extent, int extent_x, int extent_y, int extent_z, int share);
+void set_insertion_tuning(int extent, int taps, int extent_x, int extent_y, int extent_z, int share);
 void get_insertion_tuning(int extent, int *extent_x, int *extent_y, int *extent_z, int *share);
 void clear_insertion_tuning();
 

@@ -38,9 +38,15 @@ def load_table(path: str = TABLE) -> dict:
         return json.load(f)
 
 
-def set_tuning(extent: int, shape, share: int) -> None:
-    """The library's entry for volumes of extent ``extent`` (0: every other extent)."""
+def set_tuning(extent: int, shape, share: int, taps: int = 0) -> None:
+    """The library's entry for volumes of extent ``extent`` (0: every other
+    extent); with ``taps`` (1 nearest, 2 trilinear, 4 Kaiser-Bessel) the
+    entry for that window family only."""
     from . import _native as N
+    if taps:
+        N.check(N.library().rexlib_hip_set_insertion_tuning_for_window(
+            int(extent), int(taps), int(shape[0]), int(shape[1]), int(shape[2]), int(share)))
+        return
     N.check(N.library().rexlib_hip_set_insertion_tuning(int(extent), int(shape[0]), int(shape[1]), int(shape[2]),
                                                          int(share)))
 
@@ -59,10 +65,13 @@ def clear_tuning() -> None:
 
 
 def apply_all(path: str = TABLE) -> dict:
-    """Hand every recorded entry (keyed by N) to the library; returns them."""
+    """Hand every recorded entry to the library; returns them. Keys are N
+    ("256"), or N and the window's taps per axis ("256/4")."""
     table = load_table(path)
     for key, entry in table.items():
-        set_tuning(int(key), (entry["extent_x"], entry["extent_y"], entry["extent_z"]), entry["share"])
+        extent, _, taps = key.partition("/")
+        set_tuning(int(extent), (entry["extent_x"], entry["extent_y"], entry["extent_z"]), entry["share"],
+                   int(taps) if taps else 0)
     return table
 
 

@@ -206,6 +206,29 @@ def test_every_brick_shape_with_one_and_four_taps(window, tuning_table):
         tune.clear_tuning()
 
 
+@pytest.mark.parametrize("window,taps", [(oracle.Window.nearest(), 1), (oracle.Window.kaiser_bessel(2.0, 8.0), 4),
+                                         (oracle.Window.linear(), 2)])
+def test_tuning_entry_for_one_window_family(window, taps, tuning_table):
+    # A window family's own entry (here 8 x 6 x 6, share 1,024) beside a
+    # different entry for the extent: the call takes its window's entry and
+    # agrees with the reference's float32 kernels.
+    big = 96
+    inputs = random_ctf_case(big, big, 80, seed=11)
+    ref = reference_insert(inputs, window)
+    t = [to_device(a) for a in inputs]
+    try:
+        tune.set_tuning(big, (16, 16, 8), 64)
+        tune.set_tuning(big, (8, 6, 6), 1024, taps=taps)
+        num, den = zeros(big)
+        em.insert_ctf_slices(num, den, *t[2:], hip_window(window), OPTICS, 1.5, numerator_out=num,
+                             denominator_out=den)
+        torch.cuda.synchronize()
+        assert oracle.rel_l2(num.cpu().numpy(), ref[0]) <= 1e-4
+        assert oracle.rel_l2(den.cpu().numpy(), ref[1]) <= 1e-4
+    finally:
+        tune.clear_tuning()
+
+
 def padded(images, n):
     m = images.shape[-1]
     o = n // 2 - m // 2

@@ -223,3 +223,25 @@ def test_insertion_tuning_is_a_table_per_volume_extent():
     finally:
         tune.clear_tuning()
     assert tune.get_tuning(256) == ((16, 8, 6), 256)
+
+
+def test_insertion_tuning_for_one_window_family():
+    """Entries for one window family (taps per axis 1, 2 or 4) sit beside the
+    extent's entry and leave it as it is; other taps are refused; the
+    shipped table, which holds such an entry, applies whole."""
+    from rexlib_amd import tune
+    lib = N.library()
+    tune.clear_tuning()
+    try:
+        tune.set_tuning(256, (8, 8, 8), 1024)
+        tune.set_tuning(256, (8, 6, 6), 1024, taps=4)
+        assert tune.get_tuning(256) == ((8, 8, 8), 1024)
+        assert lib.rexlib_hip_set_insertion_tuning_for_window(256, 3, 8, 6, 6, 1024) == N.INVALID_ARGUMENT
+        assert lib.rexlib_hip_set_insertion_tuning_for_window(256, 4, 8, 6, 5, 1024) == N.INVALID_ARGUMENT
+        assert lib.rexlib_hip_set_insertion_tuning_for_window(1 << 20, 4, 8, 6, 6, 1024) == N.INVALID_ARGUMENT
+        table = tune.apply_all()
+        assert "256/4" in table
+        assert tune.get_tuning(256) == ((table["256"]["extent_x"], table["256"]["extent_y"],
+                                         table["256"]["extent_z"]), table["256"]["share"])
+    finally:
+        tune.clear_tuning()
