@@ -89,7 +89,18 @@ def workload(n: int, big: int, launch: int, seed: int = 0):
     return slices, transforms, ctf
 
 
-def time_insertion(n, big, slices, transforms, ctf, repeats=3):
+def window_of(taps: int):
+    """The window the tuner times for a family of `taps` per axis (0 and 2:
+    trilinear; 1: nearest; 4: Kaiser-Bessel of half-width 2, shape 7.4)."""
+    from . import em
+    if taps == 1:
+        return em.GriddingWindow.nearest()
+    if taps == 4:
+        return em.GriddingWindow.kaiser_bessel(2.0, 7.4)
+    return em.GriddingWindow.linear()
+
+
+def time_insertion(n, big, slices, transforms, ctf, repeats=3, taps=0):
     """Best time of a CTF insertion into zeroed accumulators, and its sums."""
     import torch
 
@@ -98,7 +109,7 @@ def time_insertion(n, big, slices, transforms, ctf, repeats=3):
     num = torch.zeros((big, big, big // 2 + 1), dtype=torch.complex64, device="cuda")
     den = torch.zeros((big, big, big // 2 + 1), dtype=torch.float32, device="cuda")
     radial = torch.ones(h, device="cuda")
-    window = em.GriddingWindow.linear()
+    window = window_of(taps)
     optics = em.CtfOptics(300.0, 2.7, 0.1)
 
     def run():
@@ -125,7 +136,9 @@ def rel_l2(a, b) -> float:
     return float(torch.linalg.vector_norm(a - b) / torch.clamp(torch.linalg.vector_norm(b), min=1e-30))
 
 
-def tune(n: int, big: int, launch: int, path: str = TABLE, verbose: bool = True) -> dict:
+def tune(n: int, big: int, launch: int, path: str = TABLE, verbose: bool = True, taps: int = 0) -> dict:
+    """Time every brick shape and share; record the fastest under "N" (taps
+    0: every window, timed with trilinear) or "N/taps" (one window family)."""
     slices, transforms, ctf = workload(n, big, launch)
     results = []
     reference = None
@@ -133,8 +146,8 @@ def tune(n: int, big: int, launch: int, path: str = TABLE, verbose: bool = True)
     try:
         for shape in SHAPES:
             for share in SHARES:
-                set_tuning(big, shape, share)
-                ms, sums = time_insertion(n, big, slices, transforms, ctf)
+                set_tuning(big, shape, share, taps)
+                ms, sums = time_insertion(n, big, slices, transforms, ctf, taps=taps)
                 if reference is None:
                     reference = sums
                 error = max(rel_l2(sums[0], reference[0]), rel_l2(sums[1], reference[1]))
@@ -150,13 +163,13 @@ def tune(n: int, big: int, launch: int, path: str = TABLE, verbose: bool = True)
         set_tuning(big, *previous)
     best = min(results, key=lambda r: r["ms"])
     table = load_table(path)
-    table[str(big)] = {"extent_x": best["extent_x"], "extent_y": best["extent_y"], "extent_z": best["extent_z"],
-                       "share": best["share"], "n": n,
-                       "launch": launch, "ms": best["ms"], "images_per_s": launch / (best["ms"] * 1e-3),
-                       "tuned": time.strftime("%Y-%m-%d")}
+    table[f"{big}/{taps}" if taps else str(big)] = {
+        "extent_x": best["extent_x"], "extent_y": best["extent_y"], "extent_z": best["extent_z"],
+        "share": best["share"], "n": n, "launch": launch, "ms": best["ms"],
+        "images_per_s": launch / (best["ms"] * 1e-3), "tuned": time.strftime("%Y-%m-%d")}
     with open(path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
-    set_tuning(big, (best["extent_x"], best["extent_y"], best["extent_z"]), best["share"])
+    set_tuning(big, (best["extent_x"], best["extent_y"], best["extent_z"]), best["share"], taps)
     return {"best": best, "all": results}
 
 
@@ -166,9 +179,11 @@ def main():
     parser.add_argument("--N", type=int, nargs="+", default=[256])
     parser.add_argument("--launch", type=int, default=4096)
     parser.add_argument("--table", default=TABLE)
+    parser.add_argument("--taps", type=int, default=0, choices=(0, 1, 2, 4),
+                        help="tune one window family (1 nearest, 2 trilinear, 4 Kaiser-Bessel); 0: every window")
     args = parser.parse_args()
     for n, big in zip(args.n, args.N):
-        out = tune(n, big, args.launch, args.table)
+        out = tune(n, big, args.launch, args.table, taps=args.taps)
         print(json.dumps(out["best"]), flush=True)
 
 
